@@ -1,0 +1,193 @@
+/*
+ * rxhip.h -- C ABI of librxhip.so, the MI355X (gfx950) path-tracing hot path
+ * of RenderX: KD-tree ray traversal with triangle intersection and the
+ * PathTracer bounce loop, as hand-written HIP kernels.
+ *
+ * Every entry point returns RX_OK (0) or a negative rx_status; none throws or
+ * exits.  A context is bound to one device and one HIP stream and is not
+ * thread-safe (it mirrors the reference's single static GL state,
+ * include/IntersectionGPU.h:11-45).  All host pointers are read or written
+ * synchronously unless the entry point name ends in _dev, in which case the
+ * pointers are device pointers and the work is queued on the context stream.
+ *
+ * Reference citations are file:line in the RenderX tree.
+ */
+#ifndef RXHIP_H
+#define RXHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum rx_status {
+  RX_OK = 0,
+  RX_ERR_INVALID = -1,   /* bad argument or size */
+  RX_ERR_HIP = -2,       /* a HIP runtime call failed */
+  RX_ERR_NO_MEMORY = -3, /* device or host allocation failed */
+  RX_ERR_STATE = -4,     /* call order (e.g. render before load_scene) */
+  RX_ERR_NO_DEVICE = -5, /* no usable gfx950 device */
+  RX_ERR_OVERFLOW = -6   /* a device queue or traversal stack overflowed */
+};
+
+/* object types, one per SceneObject subclass on the path */
+enum rx_object_type {
+  RX_OBJ_DIFFUSE = 0,    /* SceneDiffuseObject (include/SceneDiffuseObject.h:14-19) */
+  RX_OBJ_GLOSSY = 1,     /* SceneGlossyObject (include/SceneGlossyObject.h:12-17) */
+  RX_OBJ_PHONG = 2,      /* ScenePhongObject = glossy + diffuse lobe (include/ScenePhongObject.h:15-22) */
+  RX_OBJ_REFRACTIVE = 3, /* SceneRefractiveObject (include/SceneRefractiveObject.h:12-20) */
+  RX_OBJ_EMISSIVE = 4,   /* SceneEmissiveObject (include/SceneEmissiveObject.h) */
+  RX_OBJ_VPM = 5         /* SceneVPMObject, homogeneous medium (include/SceneVPMObject.h:8-104) */
+};
+
+/* One scene object: a contiguous range of the global triangle list plus its
+ * material (the setters of include/GlossyMaterial.h:16-25,
+ * include/SceneVPMObject.h:84-104, include/RefractiveMaterial.h).  64 bytes. */
+typedef struct rx_object_desc {
+  int32_t type;          /* rx_object_type */
+  uint32_t tri_begin;    /* first global triangle of the object */
+  uint32_t tri_count;
+  uint32_t pad0;
+  float color[3];        /* diffuse rho | glossy/phong ks | refractive surfColor | emissive Le | vpm sigma_s */
+  float exponent;        /* glossy/phong Phong exponent n */
+  float color2[3];       /* phong kd | refractive decayColor | vpm sigma_a */
+  float ior;             /* refractive refrCoeff | vpm IOR ("inrate") */
+  float g;               /* vpm Henyey-Greenstein g */
+  float pad1[3];
+} rx_object_desc;
+
+/* Pinhole camera (include/Camera.h:10-31): integer focal length in pixels. */
+typedef struct rx_camera_desc {
+  float position[3];
+  float focus[3];
+  float up[3];
+  uint32_t sight_dist;
+  int32_t width;
+  int32_t height;
+} rx_camera_desc;
+
+enum rx_render_mode {
+  RX_MODE_PT = 0,      /* PathTracer: BSDF sampling only, emitter-terminated paths (src/PathTracer.cpp:17-89) */
+  RX_MODE_NEE = 1,     /* next-event estimation to area lights + MIS, unbiased w.r.t. RX_MODE_PT */
+  RX_MODE_GUIDED = 2   /* RX_MODE_NEE + learned vMF-mixture proposal combined by one-sample MIS */
+};
+
+typedef struct rx_render_params {
+  uint32_t spp;
+  uint32_t max_depth;      /* MCRenderer::maxDepth: bounces whose outgoing ray is traced */
+  uint32_t rr_start_depth; /* 0 = Russian roulette at every bounce (the reference) */
+  uint32_t mode;           /* rx_render_mode */
+  uint64_t seed;
+  uint32_t sample_offset;  /* first sample index: resumes a render with the same counters */
+  uint32_t flags;          /* reserved, 0 */
+} rx_render_params;
+
+typedef struct rx_render_stats {
+  uint64_t rays;           /* all traversal queries: camera, bounce, shadow, containment */
+  uint64_t segments;       /* shaded path segments */
+  uint64_t samples;        /* camera paths */
+  double ms_total;         /* wall time of the render on the device stream */
+  double ms_extend;        /* summed extend (nearest-hit) kernel time */
+  double ms_shade;
+  double ms_shadow;
+  uint64_t extend_launches;
+} rx_render_stats;
+
+typedef struct rx_ctx rx_ctx;
+typedef struct rx_tree rx_tree;
+
+/* ------------------------------------------------------------------ host tree */
+enum rx_tree_flags {
+  RX_TREE_ABS_DOUBLE = 0, /* split criterion |ratio-0.5| in double (libc++/MSVC binding of abs) */
+  RX_TREE_ABS_INT = 1     /* int abs(int) binding under libstdc++: x axis only (src/KDTree.cpp:191) */
+};
+
+/* KDTree::build(BEST) -> splitBest (src/KDTree.cpp:230-246,129-228).
+ * verts: 3*n_verts floats (world positions, KDTree::vertexPositionList);
+ * tris: 3*n_tris global vertex indices (KDTree::triangleList). */
+int rx_tree_build(uint32_t n_verts, const float* verts, uint32_t n_tris, const uint32_t* tris,
+                  uint32_t flags, rx_tree** out);
+void rx_tree_free(rx_tree* tree);
+int rx_tree_info(const rx_tree* tree, uint32_t* n_nodes, uint32_t* n_leaf_tris, uint32_t* max_depth);
+/* KDTree::serializeForGPU (src/KDTree.cpp:346-383): pre-order nodes,
+ * nodes4[i] = (parent, right child, leafStart, leafEnd) as floats, min4/max4
+ * node boxes, v1/v2/v3 leaf triangle vertices with v1.w = triangle id.
+ * Sizes: nodes4/min4/max4 4*n_nodes, v1/v2/v3 4*n_leaf_tris floats. */
+int rx_tree_serialize(const rx_tree* tree, float* nodes4, float* min4, float* max4, float* v1, float* v2,
+                      float* v3);
+
+/* ------------------------------------------------------------------ context */
+/* IntersectionGPU::init (src/IntersectionGPU.cpp:16-44) */
+int rx_create(int device, rx_ctx** out);
+void rx_destroy(rx_ctx* ctx);
+const char* rx_last_error(const rx_ctx* ctx);
+int rx_synchronize(rx_ctx* ctx);
+
+/* IntersectionGPU::loadKDTreeToShader (src/IntersectionGPU.cpp:243-255):
+ * takes the arrays KDTree::serializeForGPU produces, unchanged. */
+int rx_load_tree(rx_ctx* ctx, uint32_t n_nodes, const float* nodes4, const float* min4, const float* max4,
+                 uint32_t n_leaf_tris, const float* v1, const float* v2, const float* v3);
+
+/* Per-triangle object id and face normal (SimpleShape::getWorldNormal flat
+ * branch, src/SimpleShape.cpp:393-421), indexed by triangle id; enables the
+ * NoSelfIntersectionCondition legality test (src/NoSelfIntersectionCondition.cpp:12-27). */
+int rx_load_legality(rx_ctx* ctx, uint32_t n_tris, const int32_t* tri_obj, const float* tri_normal);
+
+enum rx_intersect_flags {
+  RX_ISECT_ORDERED = 0,     /* nearer child first, conservative distance culling */
+  RX_ISECT_EXHAUSTIVE = 1,  /* every child whose box is hit: the reference recursion's visits */
+  RX_ISECT_LEGALITY = 2     /* apply NoSelfIntersectionCondition with (inside, contact) per ray */
+};
+
+/* IntersectionGPU::query (src/IntersectionGPU.cpp:216-241) with the result of
+ * KDTree::intersect(ray, triangleID, condition) (src/KDTree.cpp:335-339):
+ * org/dir 3n floats; inside/contact n object ids (-1 = none), used with
+ * RX_ISECT_LEGALITY, else may be NULL.  Outputs: dist (-1 = miss), tid (-1 =
+ * miss), optional bary (2n, FP32 of an FP64 solve: p = (1-u-v)v0+u v1+v v2),
+ * optional counts (2n: box tests, triangle tests). */
+int rx_intersect(rx_ctx* ctx, uint32_t n, const float* org, const float* dir, const int32_t* inside,
+                 const int32_t* contact, uint32_t flags, float* dist, int32_t* tid, float* bary,
+                 uint32_t* counts);
+/* Same with device pointers, queued on the context stream. */
+int rx_intersect_dev(rx_ctx* ctx, uint32_t n, const float* org, const float* dir, const int32_t* inside,
+                     const int32_t* contact, uint32_t flags, float* dist, int32_t* tid, float* bary,
+                     uint32_t* counts);
+/* Duration in ms of the last rx_intersect(_dev) traversal kernel (HIP events
+ * on the context stream; valid after rx_synchronize). */
+int rx_last_kernel_ms(rx_ctx* ctx, float* ms);
+
+/* ------------------------------------------------------------------ scene + render */
+/* Scene::buildKDTree + Scene::buildObjKDTrees (src/Scene.cpp:248-320) with
+ * the objects' materials; builds the global tree and one tree per medium,
+ * computes legality data and the emitter sampler (src/Scene.cpp:19-33,193-204). */
+int rx_load_scene(rx_ctx* ctx, uint32_t n_verts, const float* verts, uint32_t n_tris, const uint32_t* tris,
+                  uint32_t n_objects, const rx_object_desc* objects, uint32_t tree_flags);
+
+/* MCRenderer::renderPixels / PathTracer::renderPixels (include/MCRenderer.h:145-148,
+ * src/PathTracer.cpp:6-89): renders spp samples per pixel of the camera's
+ * image.  Outputs are row-major y*W+x RGB like vector<vec3f>: mean (3WH
+ * floats, may be NULL), and FP64 per-pixel sum and sum of squares (3WH
+ * doubles each, may be NULL) for variance estimates.  The film is
+ * deterministic: identical outputs for identical inputs. */
+int rx_render(rx_ctx* ctx, const rx_camera_desc* camera, const rx_render_params* params, float* rgb_mean,
+              double* sum, double* sumsq, rx_render_stats* stats);
+
+/* ------------------------------------------------------------------ path guiding */
+typedef struct rx_guide_params {
+  uint32_t train_passes;     /* warm-up passes that train the network online */
+  uint32_t samples_per_pass; /* radiance samples per training step */
+  float learning_rate;       /* Adam */
+  float beta1, beta2, eps;
+  uint32_t seed;
+  uint32_t flags;
+} rx_guide_params;
+
+/* Configures and (re)initialises the guiding network used by RX_MODE_GUIDED. */
+int rx_guide_init(rx_ctx* ctx, const rx_guide_params* params);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RXHIP_H */

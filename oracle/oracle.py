@@ -1,0 +1,174 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker, and by bench.py's cpu_baseline leg.  The product package
+renderx_amd never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+RENDER_PARAMS = np.dtype([("spp", "<u4"), ("max_depth", "<u4"), ("rr_start_depth", "<u4"), ("mode", "<u4"),
+                          ("seed", "<u8"), ("sample_offset", "<u4"), ("flags", "<u4")])
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_tree_create.restype = P
+        L.orc_tree_create.argtypes = [C.c_int, P, C.c_int, P, C.c_int]
+        L.orc_tree_free.argtypes = [P]
+        L.orc_tree_sizes.argtypes = [P, P, P]
+        L.orc_tree_serialize.argtypes = [P] * 7
+        L.orc_face_normals.argtypes = [C.c_int, P, C.c_int, P, P]
+        L.orc_intersect.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, C.c_int]
+        L.orc_barycentrics.argtypes = [C.c_int, P, P, P, P, P, P]
+        L.orc_scene_create.restype = P
+        L.orc_scene_create.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P, P, C.c_int]
+        L.orc_scene_free.argtypes = [P]
+        L.orc_scene_tree.restype = P
+        L.orc_scene_tree.argtypes = [P]
+        L.orc_render.argtypes = [P, P, P, P, C.c_int]
+        L.orc_mt_nth.restype = C.c_uint32
+        L.orc_mt_nth.argtypes = [C.c_uint32]
+        L.orc_max_threads.restype = C.c_int
+        L.orc_sampler_draws.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Tree:
+    """KDTree restatement built from vertices/triangles (KDTree::build(BEST))."""
+
+    def __init__(self, verts, tris, abs_mode=0, _handle=None, _owner=None):
+        self._verts = np.ascontiguousarray(verts, np.float32)
+        self._tris = np.ascontiguousarray(tris, np.uint32)
+        self._owner = _owner
+        if _handle is not None:
+            self.h = _handle
+        else:
+            self.h = lib().orc_tree_create(len(self._verts), _p(self._verts), len(self._tris), _p(self._tris),
+                                           abs_mode)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self._owner is None:
+            lib().orc_tree_free(self.h)
+            self.h = None
+
+    def serialize(self):
+        """KDTree::serializeForGPU arrays: nodes, min, max, v1, v2, v3 (each (n,4) float32)."""
+        nn, nl = C.c_int(), C.c_int()
+        lib().orc_tree_sizes(self.h, C.byref(nn), C.byref(nl))
+        out = [np.zeros((nn.value, 4), np.float32) for _ in range(3)] + \
+              [np.zeros((nl.value, 4), np.float32) for _ in range(3)]
+        lib().orc_tree_serialize(self.h, *[_p(a) for a in out])
+        return out
+
+    def intersect(self, org, dirs, mode=0, tri_obj=None, tri_normal=None, inside=None, contact=None,
+                  threads=0):
+        """mode 0 exhaustive recursion, 1 ordered (canonical counts), 2 brute force."""
+        org = np.ascontiguousarray(org, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = len(org)
+        dist = np.empty(n, np.float32)
+        tid = np.empty(n, np.int32)
+        boxes = np.zeros(n, np.uint32)
+        tris = np.zeros(n, np.uint32)
+        cv = lambda a, t: None if a is None else np.ascontiguousarray(a, t)
+        tri_obj, tri_normal = cv(tri_obj, np.int32), cv(tri_normal, np.float32)
+        inside, contact = cv(inside, np.int32), cv(contact, np.int32)
+        lib().orc_intersect(self.h, mode, n, _p(org), _p(dirs), _p(tri_obj), _p(tri_normal), _p(inside),
+                            _p(contact), _p(dist), _p(tid), _p(boxes), _p(tris), threads)
+        return dist, tid, boxes, tris
+
+
+def face_normals(verts, tris):
+    verts = np.ascontiguousarray(verts, np.float32)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    out = np.empty((len(tris), 3), np.float32)
+    lib().orc_face_normals(len(verts), _p(verts), len(tris), _p(tris), _p(out))
+    return out
+
+
+def barycentrics(org, dirs, tid, verts, tris):
+    org = np.ascontiguousarray(org, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    tid = np.ascontiguousarray(tid, np.int32)
+    verts = np.ascontiguousarray(verts, np.float32)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    uv = np.empty((len(org), 2), np.float64)
+    lib().orc_barycentrics(len(org), _p(org), _p(dirs), _p(tid), _p(verts), _p(tris), _p(uv))
+    return uv
+
+
+class SceneOracle:
+    """Restated PathTracer / PathTracerTest over a scene description."""
+
+    def __init__(self, scene, abs_mode=0):
+        self.scene = scene
+        self._v = np.ascontiguousarray(scene.verts, np.float32)
+        self._t = np.ascontiguousarray(scene.tris, np.uint32)
+        self._o = np.ascontiguousarray(scene.objects)
+        self._c = np.ascontiguousarray(scene.camera.reshape(1))
+        self.h = lib().orc_scene_create(len(self._v), _p(self._v), len(self._t), _p(self._t), len(self._o),
+                                        _p(self._o), _p(self._c), abs_mode)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_free(self.h)
+            self.h = None
+
+    def tree(self) -> Tree:
+        return Tree(self._v, self._t, _handle=lib().orc_scene_tree(self.h), _owner=self)
+
+    def render(self, spp, max_depth=None, rr_start_depth=None, mode=0, seed=1, sample_offset=0, threads=0):
+        """FP64 per-pixel (sum, sum of squares), shape (H, W, 3)."""
+        cam = self.scene.camera
+        H, W = int(cam["height"]), int(cam["width"])
+        p = np.zeros((), RENDER_PARAMS)
+        p["spp"] = spp
+        p["max_depth"] = self.scene.max_depth if max_depth is None else max_depth
+        p["rr_start_depth"] = self.scene.rr_start_depth if rr_start_depth is None else rr_start_depth
+        p["mode"] = mode
+        p["seed"] = seed
+        p["sample_offset"] = sample_offset
+        p = p.reshape(1)
+        s = np.zeros((H, W, 3), np.float64)
+        q = np.zeros((H, W, 3), np.float64)
+        lib().orc_render(self.h, _p(p), _p(s), _p(q), threads)
+        return s, q
+
+
+def mt_nth(n: int) -> int:
+    return int(lib().orc_mt_nth(n))
+
+
+def sampler_draws(kind: int, n: int, param: float, param2: float = 0.0, seed: int = 1):
+    """kind 0 cosine lobe, 1 HG phase, 2 medium distance, 3 Fresnel er; (n,3) float32."""
+    out = np.empty((n, 3), np.float32)
+    lib().orc_sampler_draws(kind, n, param, param2, seed, _p(out))
+    return out

@@ -1,0 +1,120 @@
+// Oracle: CPU restatement of RenderX's KD-tree build, serialization, box test,
+// triangle test and exhaustive nearest-hit recursion.
+//
+// TEST INFRASTRUCTURE ONLY.  This code is the checker for the HIP path of
+// renderx_amd; only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it.  Nothing in renderx_amd/ links or calls it.
+//
+// Every function restates the semantics of a reference function (file:line
+// relative to the RenderX tree).  Floating-point operation order follows the
+// reference's vec3 template (include/nvVector.h:366-395 for +,-,*,/;
+// :491-527 for cross/dot/length/normalize): dot is ((0+a0*b0)+a1*b1)+a2*b2,
+// normalize divides each component by the correctly rounded length.  The file
+// is compiled with -ffp-contract=off so no FMA is formed.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+namespace orc {
+
+struct V3 {
+  float x, y, z;
+  V3() : x(0), y(0), z(0) {}
+  V3(float a, float b, float c) : x(a), y(b), z(c) {}
+  float& operator[](int i) { return (&x)[i]; }
+  float operator[](int i) const { return (&x)[i]; }
+};
+inline V3 operator+(V3 a, V3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator-(V3 a) { return V3(-a.x, -a.y, -a.z); }
+// vec*scalar and scalar*vec: component-wise lhs[i]*rhs  (nvVector.h:~420-432)
+inline V3 operator*(V3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+inline V3 operator*(float s, V3 a) { return V3(s * a.x, s * a.y, s * a.z); }
+inline V3 operator*(V3 a, V3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+// operator/ with scalar: skip when d == 0 (nvVector.h operator/=)
+inline V3 operator/(V3 a, float d) {
+  if (d == 0) return a;
+  return V3(a.x / d, a.y / d, a.z / d);
+}
+inline V3 cross(V3 l, V3 r) {  // nvVector.h:491-500
+  return V3(l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x);
+}
+inline float dot(V3 a, V3 b) {  // nvVector.h:502-508
+  float r = 0.0f;
+  r += a.x * b.x;
+  r += a.y * b.y;
+  r += a.z * b.z;
+  return r;
+}
+inline float length(V3 a) {  // nvVector.h:511-519: sqrt of float square_norm
+  float r = 0.0f;
+  r += a.x * a.x;
+  r += a.y * a.y;
+  r += a.z * a.z;
+  return std::sqrt(r);
+}
+inline V3 normalize(V3 a) {  // nvVector.h:523-527
+  float r = length(a);
+  if (r > 0) return a / r;
+  return a;
+}
+
+// KDTree::Node (include/KDTree.h:61-72) as an index tree.
+struct Node {
+  int left = -1, right = -1;
+  std::vector<unsigned> tris;  // triangleIndices (only kept for leaves)
+  V3 lo, hi;                   // boundingBox.minCoord / maxCoord
+};
+
+// abs semantics of the split criterion `abs(ratio-0.5)` (src/KDTree.cpp:191):
+//   ABS_DOUBLE  -- abs(double), libc++ / MSVC (the author's toolchains)
+//   ABS_INT     -- int abs(int) as bound under libstdc++: |.| truncates to 0
+enum AbsMode { ABS_DOUBLE = 0, ABS_INT = 1 };
+
+struct Tree {
+  std::vector<V3> verts;         // KDTree::vertexPositionList
+  std::vector<uint32_t> tri_v;   // 3 per triangle (KDTree::triangleList[i].vertexIndices)
+  std::vector<Node> nodes;       // nodes[0] = root
+  int abs_mode = ABS_DOUBLE;
+};
+
+void build_tree(Tree& t);  // KDTree::build(BEST) src/KDTree.cpp:230-246
+
+// KDTree::serializeForGPU (src/KDTree.cpp:346-383): pre-order, left = i+1.
+struct Serialized {
+  std::vector<float> nodes4, min4, max4, v1_4, v2_4, v3_4;
+};
+void serialize(const Tree& t, Serialized& s);
+
+struct Ray {
+  V3 o, d;
+};
+
+// KDTree::Node::BoundingBox::intersect (src/KDTree.cpp:34-52)
+float box_intersect(V3 lo, V3 hi, const Ray& r);
+// KDTree::intersect(ray, triangle, condition) (src/KDTree.cpp:267-293) without the condition
+float tri_intersect(V3 v0, V3 v1, V3 v2, const Ray& r);
+
+// NoSelfIntersectionCondition (src/NoSelfIntersectionCondition.cpp:12-27)
+struct Legality {
+  const int* tri_obj;     // object id per global triangle
+  const V3* tri_normal;   // face normal per global triangle (SimpleShape::getWorldNormal)
+  int inside, contact;    // ray's insideObject / contactObject ids (-1 = NULL)
+};
+bool legal(const Legality& c, const Ray& r, unsigned tid);
+V3 face_normal(V3 v0, V3 v1, V3 v2);  // SimpleShape::getWorldNormal flat branch (src/SimpleShape.cpp:393-421)
+
+struct HitStats {
+  uint32_t boxes = 0, tris = 0;
+};
+// KDTree::intersect recursion (src/KDTree.cpp:295-339), exhaustive.
+float intersect_exhaustive(const Tree& t, const Ray& r, unsigned& tid,
+                           const Legality* cond, HitStats* st);
+// Ordered traversal with conservative culling: the canonical visit counts.
+float intersect_ordered(const Tree& t, const Ray& r, unsigned& tid,
+                        const Legality* cond, HitStats* st);
+float intersect_bruteforce(const Tree& t, const Ray& r, unsigned& tid,
+                           const Legality* cond);
+
+}  // namespace orc

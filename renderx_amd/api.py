@@ -1,0 +1,162 @@
+"""Python mirror of the reference's interface for the hot path.
+
+  KDTree           -- KDTree::build / serializeForGPU (include/KDTree.h:10-59)
+  IntersectionGPU  -- IntersectionGPU::init / loadKDTreeToShader / query
+                      (include/IntersectionGPU.h:11-45), answered by HIP kernels
+  Scene            -- Scene::buildKDTree + objects (include/Scene.h:10-107)
+  PathTracerHIP    -- MCRenderer::renderPixels (include/MCRenderer.h:145-148)
+
+All calls go through the C ABI of librxhip.so (include/rxhip.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr
+
+ORDERED, EXHAUSTIVE, LEGALITY = 0, 1, 2
+
+RENDER_PARAMS = np.dtype([("spp", "<u4"), ("max_depth", "<u4"), ("rr_start_depth", "<u4"), ("mode", "<u4"),
+                          ("seed", "<u8"), ("sample_offset", "<u4"), ("flags", "<u4")])
+RENDER_STATS = np.dtype([("rays", "<u8"), ("segments", "<u8"), ("samples", "<u8"), ("ms_total", "<f8"),
+                         ("ms_extend", "<f8"), ("ms_shade", "<f8"), ("ms_shadow", "<f8"),
+                         ("extend_launches", "<u8")])
+GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("samples_per_pass", "<u4"), ("learning_rate", "<f4"),
+                         ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4")])
+MODE_PT, MODE_NEE, MODE_GUIDED = 0, 1, 2
+
+
+class KDTree:
+    """Host KD-tree with the reference's median split (KDTree::build(BEST))."""
+
+    def __init__(self, vertexPositionList, triangleList, abs_int: bool = False):
+        self.verts = np.ascontiguousarray(vertexPositionList, np.float32).reshape(-1, 3)
+        self.tris = np.ascontiguousarray(triangleList, np.uint32).reshape(-1, 3)
+        h = C.c_void_p()
+        check(None, lib().rx_tree_build(len(self.verts), ptr(self.verts), len(self.tris), ptr(self.tris),
+                                        1 if abs_int else 0, C.byref(h)), "rx_tree_build")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().rx_tree_free(self.h)
+            self.h = None
+
+    def info(self):
+        n, m, d = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(None, lib().rx_tree_info(self.h, C.byref(n), C.byref(m), C.byref(d)), "rx_tree_info")
+        return n.value, m.value, d.value
+
+    def serializeForGPU(self):
+        n, m, _ = self.info()
+        out = [np.zeros((n, 4), np.float32) for _ in range(3)] + [np.zeros((m, 4), np.float32) for _ in range(3)]
+        check(None, lib().rx_tree_serialize(self.h, *[ptr(a) for a in out]), "rx_tree_serialize")
+        return out
+
+
+class Context:
+    """One device + stream (rx_create)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(None, lib().rx_create(device, C.byref(h)), "rx_create")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().rx_destroy(self.h)
+            self.h = None
+
+    def check(self, code, what):
+        check(self.h, code, what)
+
+
+class IntersectionGPU:
+    """Drop-in for the reference's GL intersection seam, on HIP."""
+
+    def __init__(self, device: int = 0, ctx: Context | None = None):
+        self.ctx = ctx or Context(device)
+
+    def loadKDTreeToShader(self, tree_or_arrays):
+        arrs = tree_or_arrays.serializeForGPU() if isinstance(tree_or_arrays, KDTree) else tree_or_arrays
+        nodes, mn, mx, v1, v2, v3 = [np.ascontiguousarray(a, np.float32) for a in arrs]
+        self.ctx.check(lib().rx_load_tree(self.ctx.h, len(nodes), ptr(nodes), ptr(mn), ptr(mx), len(v1), ptr(v1),
+                                          ptr(v2), ptr(v3)), "rx_load_tree")
+
+    def load_legality(self, tri_obj, tri_normal=None):
+        tri_obj = np.ascontiguousarray(tri_obj, np.int32)
+        tn = None if tri_normal is None else np.ascontiguousarray(tri_normal, np.float32)
+        self.ctx.check(lib().rx_load_legality(self.ctx.h, len(tri_obj), ptr(tri_obj), ptr(tn)), "rx_load_legality")
+
+    def query(self, org, dirs, inside=None, contact=None, exhaustive=False, bary=False, counts=False):
+        org = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = len(org)
+        flags = (EXHAUSTIVE if exhaustive else ORDERED)
+        ins = con = None
+        if inside is not None:
+            flags |= LEGALITY
+            ins = np.ascontiguousarray(inside, np.int32)
+            con = np.ascontiguousarray(contact, np.int32)
+        dist = np.empty(n, np.float32)
+        tid = np.empty(n, np.int32)
+        b = np.empty((n, 2), np.float32) if bary else None
+        c = np.empty((n, 2), np.uint32) if counts else None
+        self.ctx.check(lib().rx_intersect(self.ctx.h, n, ptr(org), ptr(dirs), ptr(ins), ptr(con), flags, ptr(dist),
+                                          ptr(tid), ptr(b), ptr(c)), "rx_intersect")
+        out = {"dist": dist, "tid": tid}
+        if bary:
+            out["bary"] = b
+        if counts:
+            out["counts"] = c
+        return out
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float()
+        self.ctx.check(lib().rx_last_kernel_ms(self.ctx.h, C.byref(ms)), "rx_last_kernel_ms")
+        return ms.value
+
+
+class PathTracerHIP:
+    """MCRenderer plugin: renderPixels(camera) on the wavefront HIP renderer."""
+
+    def __init__(self, scene, device: int = 0, ctx: Context | None = None, tree_abs_int: bool = False):
+        self.ctx = ctx or Context(device)
+        self.scene = scene
+        v = np.ascontiguousarray(scene.verts, np.float32)
+        t = np.ascontiguousarray(scene.tris, np.uint32)
+        o = np.ascontiguousarray(scene.objects)
+        self.ctx.check(lib().rx_load_scene(self.ctx.h, len(v), ptr(v), len(t), ptr(t), len(o), ptr(o),
+                                           1 if tree_abs_int else 0), "rx_load_scene")
+
+    def guide_init(self, train_passes=8, samples_per_pass=1 << 18, learning_rate=1e-2, seed=7):
+        p = np.zeros(1, GUIDE_PARAMS)
+        p["train_passes"] = train_passes
+        p["samples_per_pass"] = samples_per_pass
+        p["learning_rate"] = learning_rate
+        p["beta1"], p["beta2"], p["eps"] = 0.9, 0.999, 1e-8
+        p["seed"] = seed
+        self.ctx.check(lib().rx_guide_init(self.ctx.h, ptr(p)), "rx_guide_init")
+
+    def renderPixels(self, camera=None, spp=None, max_depth=None, rr_start_depth=None, mode=MODE_PT, seed=1,
+                     sample_offset=0, want_sums=True):
+        """Returns dict(mean=(H,W,3) f32, sum, sumsq=(H,W,3) f64, stats)."""
+        cam = np.ascontiguousarray((self.scene.camera if camera is None else camera).reshape(1))
+        H, W = int(cam["height"][0]), int(cam["width"][0])
+        p = np.zeros(1, RENDER_PARAMS)
+        p["spp"] = self.scene.spp if spp is None else spp
+        p["max_depth"] = self.scene.max_depth if max_depth is None else max_depth
+        p["rr_start_depth"] = self.scene.rr_start_depth if rr_start_depth is None else rr_start_depth
+        p["mode"] = mode
+        p["seed"] = seed
+        p["sample_offset"] = sample_offset
+        mean = np.empty((H, W, 3), np.float32)
+        s = np.empty((H, W, 3), np.float64) if want_sums else None
+        q = np.empty((H, W, 3), np.float64) if want_sums else None
+        st = np.zeros(1, RENDER_STATS)
+        self.ctx.check(lib().rx_render(self.ctx.h, ptr(cam), ptr(p), ptr(mean), ptr(s), ptr(q), ptr(st)),
+                       "rx_render")
+        return {"mean": mean, "sum": s, "sumsq": q, "stats": {k: st[k][0].item() for k in RENDER_STATS.names}}

@@ -1,0 +1,264 @@
+// Device-side data layout and traversal for gfx950.
+//
+// Geometry arithmetic here must reproduce the reference's float results bit
+// for bit, so every file including this header is compiled with
+// -ffp-contract=off and correctly rounded division / square root; the
+// operation order of each expression follows RenderX's vec3 template
+// (include/nvVector.h:366-395, 491-527).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rx {
+
+// ------------------------------------------------------------------ layout
+// Interior node, 64 bytes: both children's boxes and references.
+//   f[0..5]  left child lo.xyz, hi.xyz   f[6..11] right child lo.xyz, hi.xyz
+//   i[12]    left ref                    i[13]    right ref
+// A reference >= 0 is an interior node index; < 0 is leaf ~ref.
+struct DevTree {
+  const float4* nodes;
+  const int32_t* leaf_start;  // n_leaves + 1 offsets into the leaf-ordered triangles
+  const float4* tv0;          // v0.xyz, w = triangle id bits
+  const float4* tv1;          // v1.xyz, w = object id bits
+  const float4* tv2;          // v2.xyz
+  const float4* tn;           // face normal xyz, w = object-local triangle bits
+  float root_lo[3], root_hi[3];
+  int32_t root_ref;
+  uint32_t n_leaf_tris;
+};
+
+struct Hit {
+  float dist;   // -1 = miss
+  int32_t pos;  // leaf-ordered triangle position, -1 = miss
+};
+
+struct Vec3 {
+  float x, y, z;
+};
+__device__ __forceinline__ Vec3 v3(float a, float b, float c) { return Vec3{a, b, c}; }
+__device__ __forceinline__ Vec3 operator+(Vec3 a, Vec3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ Vec3 operator-(Vec3 a, Vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ Vec3 operator-(Vec3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ Vec3 operator*(Vec3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ Vec3 operator*(float s, Vec3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ Vec3 operator*(Vec3 a, Vec3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ Vec3 cross(Vec3 l, Vec3 r) {
+  return v3(l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x);
+}
+// ((0 + a.x*b.x) + a.y*b.y) + a.z*b.z: the leading 0 + turns -0 into +0 like the reference
+__device__ __forceinline__ float dot(Vec3 a, Vec3 b) {
+  float r = 0.0f;
+  r += a.x * b.x;
+  r += a.y * b.y;
+  r += a.z * b.z;
+  return r;
+}
+__device__ __forceinline__ float length(Vec3 a) { return __builtin_sqrtf(dot(a, a)); }
+__device__ __forceinline__ Vec3 normalize(Vec3 a) {
+  float r = length(a);
+  if (r > 0) return v3(a.x / r, a.y / r, a.z / r);
+  return a;
+}
+__device__ __forceinline__ Vec3 xyz(float4 f) { return v3(f.x, f.y, f.z); }
+__device__ __forceinline__ float comp(Vec3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// ------------------------------------------------------------------ predicates
+// KDTree::Node::BoundingBox::intersect (src/KDTree.cpp:34-52) reduced to the
+// only thing the reference reads from it: minDist >= 0, i.e. some face passes.
+// Faces are independent, so the loop may stop at the first passing face.
+__device__ __forceinline__ bool face_pass(float plane, float o_a, float d_a, float o_b, float d_b, float lo_b,
+                                          float hi_b, float o_c, float d_c, float lo_c, float hi_c) {
+  if (d_a == 0) return false;
+  float dist = (plane - o_a) / d_a;
+  float pb = o_b + dist * d_b;
+  float pc = o_c + dist * d_c;
+  return dist >= 0 && lo_b <= pb && hi_b >= pb && lo_c <= pc && hi_c >= pc;
+}
+__device__ __forceinline__ bool box_hit(const float lo[3], const float hi[3], Vec3 o, Vec3 d) {
+  // faces i = 0..5: plane ((float*)this)[i], axis a = i%3, checks (i+1)%3, (i+2)%3
+  if (face_pass(lo[0], o.x, d.x, o.y, d.y, lo[1], hi[1], o.z, d.z, lo[2], hi[2])) return true;
+  if (face_pass(lo[1], o.y, d.y, o.z, d.z, lo[2], hi[2], o.x, d.x, lo[0], hi[0])) return true;
+  if (face_pass(lo[2], o.z, d.z, o.x, d.x, lo[0], hi[0], o.y, d.y, lo[1], hi[1])) return true;
+  if (face_pass(hi[0], o.x, d.x, o.y, d.y, lo[1], hi[1], o.z, d.z, lo[2], hi[2])) return true;
+  if (face_pass(hi[1], o.y, d.y, o.z, d.z, lo[2], hi[2], o.x, d.x, lo[0], hi[0])) return true;
+  return face_pass(hi[2], o.z, d.z, o.x, d.x, lo[0], hi[0], o.y, d.y, lo[1], hi[1]);
+}
+
+// Slab entry distance with per-ray reciprocals: ordering key only (not a hit
+// test).  Sub then mul, compare-select; identical to the oracle's key.
+__device__ __forceinline__ float slab_tnear(const float lo[3], const float hi[3], Vec3 o, Vec3 d, Vec3 inv) {
+  const float INF = __builtin_inff();
+  float t0, t1, a, b, c;
+  if (d.x == 0) { t0 = -INF; t1 = INF; } else { t0 = (lo[0] - o.x) * inv.x; t1 = (hi[0] - o.x) * inv.x; }
+  a = t0 < t1 ? t0 : t1;
+  if (d.y == 0) { t0 = -INF; t1 = INF; } else { t0 = (lo[1] - o.y) * inv.y; t1 = (hi[1] - o.y) * inv.y; }
+  b = t0 < t1 ? t0 : t1;
+  if (d.z == 0) { t0 = -INF; t1 = INF; } else { t0 = (lo[2] - o.z) * inv.z; t1 = (hi[2] - o.z) * inv.z; }
+  c = t0 < t1 ? t0 : t1;
+  float t = a;
+  t = b > t ? b : t;
+  t = c > t ? c : t;
+  return t;
+}
+
+// KDTree::intersect(ray, triangle) (src/KDTree.cpp:267-293) without the
+// condition.  The edge-sign masks decide the hit before the normalisation and
+// division, which only run for candidates (no side effects, same result).
+__device__ __forceinline__ float tri_dist(Vec3 v0, Vec3 v1, Vec3 v2, Vec3 o, Vec3 d) {
+  Vec3 l0 = v1 - v0, l1 = v2 - v1, l2 = v0 - v2;
+  Vec3 r0 = v0 - o, r1 = v1 - o, r2 = v2 - o;
+  float s0 = dot(d, cross(l0, r0));
+  float s1 = dot(d, cross(l1, r1));
+  float s2 = dot(d, cross(l2, r2));
+  bool pos = s0 >= 0 && s1 >= 0 && s2 >= 0;
+  bool neg = s0 <= 0 && s1 <= 0 && s2 <= 0;
+  if (!pos && !neg) return -1.0f;
+  Vec3 n = normalize(cross(l0, l1));
+  float orthoDist = dot(n, r2);
+  float den = dot(n, d);
+  if (den == 0) return -1.0f;
+  float dist = orthoDist / den;
+  if (dist < 0) return -1.0f;
+  return dist;  // may be NaN for degenerate triangles: the caller's dist >= 0 rejects it
+}
+
+// NoSelfIntersectionCondition::legal (src/NoSelfIntersectionCondition.cpp:12-27)
+__device__ __forceinline__ bool legal(Vec3 d, Vec3 n, int32_t obj, int32_t inside, int32_t contact) {
+  float dn = dot(d, n);
+  bool in = dn <= 0, out = dn >= 0;
+  if (inside == contact) return !(in && contact == obj);
+  return !(out && inside != obj);
+}
+
+struct TraceStats {
+  uint32_t boxes, tris;
+};
+
+// Nearest hit of the reference tree.  EXHAUSTIVE visits every child whose
+// box passes (the reference recursion's visit set); otherwise the nearer
+// child (slab key) goes first and subtrees whose key exceeds best*1.001 are
+// culled.  Both return the reference's winner: among equal distances the
+// larger DFS leaf index wins (right subtree wins, src/KDTree.cpp:317-321),
+// inside a leaf the first triangle wins (:305).
+//
+// stack: per-lane LDS column of LDS_DEPTH entries (stride = blockDim.x),
+// spill: per-thread global overflow of SPILL_DEPTH entries.
+template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH>
+__device__ __forceinline__ Hit trace(const DevTree& T, Vec3 o, Vec3 d, int32_t inside, int32_t contact,
+                                     uint2* lds_stack, uint2* spill, int spill_depth, TraceStats* st,
+                                     bool* overflow) {
+  Hit h{-1.0f, -1};
+  if (STATS) { st->boxes = 1; st->tris = 0; }
+  if (!box_hit(T.root_lo, T.root_hi, o, d)) return h;
+  const Vec3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  const float INF = __builtin_inff();
+  bool has = false;
+  float best = -1.0f;
+  int32_t bestLeaf = -1, bestPos = -1;
+  int sp = 0;
+  int32_t ref = T.root_ref;
+  const int stride = blockDim.x;
+  for (;;) {
+    // descend interior nodes
+    while (ref >= 0) {
+      const float4* np = T.nodes + 4 * (size_t)ref;
+      float4 a = np[0], b = np[1], c = np[2];
+      int4 r = *reinterpret_cast<const int4*>(np + 3);
+      const float llo[3] = {a.x, a.y, a.z}, lhi[3] = {a.w, b.x, b.y};
+      const float rlo[3] = {b.z, b.w, c.x}, rhi[3] = {c.y, c.z, c.w};
+      if (STATS) st->boxes += 2;
+      bool hl = box_hit(llo, lhi, o, d);
+      bool hr = box_hit(rlo, rhi, o, d);
+      int32_t next = INT32_MIN;
+      if (EXHAUSTIVE) {
+        if (hl && hr) {
+          if (sp < LDS_DEPTH) lds_stack[sp * stride] = make_uint2((uint32_t)r.y, 0u);
+          else if (sp - LDS_DEPTH < spill_depth) spill[sp - LDS_DEPTH] = make_uint2((uint32_t)r.y, 0u);
+          else { *overflow = true; return Hit{-1.0f, -1}; }
+          sp++;
+          next = r.x;
+        } else if (hl) next = r.x;
+        else if (hr) next = r.y;
+      } else {
+        float tl = slab_tnear(llo, lhi, o, d, inv);
+        float tr = slab_tnear(rlo, rhi, o, d, inv);
+        float thr = has ? best * 1.001f : INF;
+        bool vl = hl && tl <= thr, vr = hr && tr <= thr;
+        if (vl && vr) {
+          int32_t nearRef = (tr < tl) ? r.y : r.x;
+          int32_t farRef = (tr < tl) ? r.x : r.y;
+          float farT = (tr < tl) ? tl : tr;
+          uint2 e = make_uint2((uint32_t)farRef, __float_as_uint(farT));
+          if (sp < LDS_DEPTH) lds_stack[sp * stride] = e;
+          else if (sp - LDS_DEPTH < spill_depth) spill[sp - LDS_DEPTH] = e;
+          else { *overflow = true; return Hit{-1.0f, -1}; }
+          sp++;
+          next = nearRef;
+        } else if (vl) next = r.x;
+        else if (vr) next = r.y;
+      }
+      if (next == INT32_MIN) { ref = INT32_MIN; break; }
+      ref = next;
+    }
+    if (ref != INT32_MIN) {  // a leaf
+      const int32_t leaf = ~ref;
+      const int32_t p0 = T.leaf_start[leaf], p1 = T.leaf_start[leaf + 1];
+      for (int32_t p = p0; p < p1; p++) {
+        if (STATS) st->tris++;
+        float4 a = T.tv0[p], b = T.tv1[p], c = T.tv2[p];
+        float dist = tri_dist(xyz(a), xyz(b), xyz(c), o, d);
+        if (!(dist >= 0)) continue;
+        bool win = !has || dist < best || (dist == best && leaf > bestLeaf);
+        if (!win) continue;
+        if (LEGALITY) {
+          float4 n = T.tn[p];
+          if (!legal(d, xyz(n), __float_as_int(b.w), inside, contact)) continue;
+        }
+        has = true;
+        best = dist;
+        bestLeaf = leaf;
+        bestPos = p;
+      }
+    }
+    // pop
+    ref = INT32_MIN;
+    while (sp > 0) {
+      sp--;
+      uint2 e = (sp < LDS_DEPTH) ? lds_stack[sp * stride] : spill[sp - LDS_DEPTH];
+      if (EXHAUSTIVE || !has || __uint_as_float(e.y) <= best * 1.001f) {
+        ref = (int32_t)e.x;
+        break;
+      }
+    }
+    if (ref == INT32_MIN) break;
+  }
+  if (has) {
+    h.dist = best;
+    h.pos = bestPos;
+  }
+  return h;
+}
+
+// ------------------------------------------------------------------ RNG
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter-based, one
+// independent stream per (pixel, sample) keyed by the seed.
+struct U4 {
+  uint32_t x, y, z, w;
+};
+__device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int i = 0; i < 10; i++) {
+    uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+    uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+// 24-bit uniform in [0, 1 - 2^-24], the reference's float form (src/RandGenerator.cpp:21-25)
+__device__ __forceinline__ float u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
+
+}  // namespace rx
