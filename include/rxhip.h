@@ -80,15 +80,25 @@ typedef struct rx_render_params {
   uint32_t mode;           /* rx_render_mode */
   uint64_t seed;
   uint32_t sample_offset;  /* first sample index: resumes a render with the same counters */
-  uint32_t flags;          /* reserved, 0 */
+  uint32_t flags;          /* rx_render_flags */
 } rx_render_params;
+
+enum rx_render_flags {
+  RX_RENDER_TIMING = 1,     /* time each kernel with HIP events (ms_extend/ms_shade/ms_shadow) */
+  RX_RENDER_COUNT_WORK = 2, /* count box and triangle tests of the extend kernels */
+  RX_RENDER_NO_GRAPH = 4    /* launch kernels one by one instead of replaying a captured pass graph */
+};
 
 typedef struct rx_render_stats {
   uint64_t rays;           /* all traversal queries: camera, bounce, shadow, containment */
+  uint64_t extend_rays;    /* nearest-hit queries of the extend kernels */
+  uint64_t shadow_rays;    /* next-event shadow queries (one per boundary crossing) */
   uint64_t segments;       /* shaded path segments */
   uint64_t samples;        /* camera paths */
+  uint64_t box_tests;      /* extend traversal box tests (RX_RENDER_COUNT_WORK) */
+  uint64_t tri_tests;      /* extend traversal triangle tests (RX_RENDER_COUNT_WORK) */
   double ms_total;         /* wall time of the render on the device stream */
-  double ms_extend;        /* summed extend (nearest-hit) kernel time */
+  double ms_extend;        /* summed extend kernel time (RX_RENDER_TIMING) */
   double ms_shade;
   double ms_shadow;
   uint64_t extend_launches;
@@ -125,7 +135,8 @@ const char* rx_last_error(const rx_ctx* ctx);
 int rx_synchronize(rx_ctx* ctx);
 
 /* IntersectionGPU::loadKDTreeToShader (src/IntersectionGPU.cpp:243-255):
- * takes the arrays KDTree::serializeForGPU produces, unchanged. */
+ * takes the arrays KDTree::serializeForGPU produces, unchanged.  Replaces
+ * any loaded tree and clears legality data (load it again afterwards). */
 int rx_load_tree(rx_ctx* ctx, uint32_t n_nodes, const float* nodes4, const float* min4, const float* max4,
                  uint32_t n_leaf_tris, const float* v1, const float* v2, const float* v3);
 

@@ -101,6 +101,27 @@ void build_tree(Tree& t) {
     work.push_back(ri);
     work.push_back(li);
   }
+  // leaf order of the pre-order traversal (the serialization order)
+  t.leaf_rank.assign(t.nodes.size(), -1);
+  t.tri_leaf.assign(ntri, -1);
+  t.tri_pos.assign(ntri, 0);
+  std::vector<int> stk{0};
+  int next = 0;
+  while (!t.nodes.empty() && !stk.empty()) {
+    int n = stk.back();
+    stk.pop_back();
+    if (t.nodes[n].left < 0) {
+      int p = 0;
+      for (unsigned tr : t.nodes[n].tris) {
+        t.tri_leaf[tr] = next;
+        t.tri_pos[tr] = p++;
+      }
+      t.leaf_rank[n] = next++;
+    } else {
+      stk.push_back(t.nodes[n].right);
+      stk.push_back(t.nodes[n].left);
+    }
+  }
 }
 
 static void serialize_rec(const Tree& t, int ni, int parent, Serialized& s) {
@@ -253,26 +274,9 @@ static inline float slab_tnear(V3 lo, V3 hi, const Ray& r, const float inv[3]) {
 float intersect_ordered(const Tree& t, const Ray& r, unsigned& tid, const Legality* cond,
                         HitStats* st) {
   if (t.nodes.empty()) return -1;
-  // leaf DFS rank: leaves numbered in pre-order; larger rank wins distance ties
-  // (right subtree wins, src/KDTree.cpp:317-321); first in leaf wins (:305).
-  static thread_local const Tree* cached = nullptr;
-  static thread_local std::vector<int> rank;
-  if (cached != &t) {
-    rank.assign(t.nodes.size(), -1);
-    std::vector<int> stk{0};
-    int next = 0;
-    while (!stk.empty()) {
-      int n = stk.back();
-      stk.pop_back();
-      if (t.nodes[n].left < 0) {
-        rank[n] = next++;
-      } else {
-        stk.push_back(t.nodes[n].right);
-        stk.push_back(t.nodes[n].left);
-      }
-    }
-    cached = &t;
-  }
+  // larger leaf rank wins distance ties (right subtree wins, src/KDTree.cpp:317-321);
+  // first in leaf wins (:305)
+  const std::vector<int>& rank = t.leaf_rank;
   const float INF = std::numeric_limits<float>::infinity();
   float inv[3];
   for (int a = 0; a < 3; a++) inv[a] = 1.0f / r.d[a];
@@ -347,26 +351,8 @@ float intersect_ordered(const Tree& t, const Ray& r, unsigned& tid, const Legali
 float intersect_bruteforce(const Tree& t, const Ray& r, unsigned& tid, const Legality* cond) {
   // Nearest legal hit over all triangles; ties broken by the tree's leaf order
   // exactly as the recursion does.
-  std::vector<int> leafOf(t.tri_v.size() / 3, -1), posOf(t.tri_v.size() / 3, 0);
-  {
-    std::vector<int> stk{0};
-    int next = 0;
-    while (!stk.empty()) {
-      int n = stk.back();
-      stk.pop_back();
-      if (t.nodes[n].left < 0) {
-        int p = 0;
-        for (unsigned tr : t.nodes[n].tris) {
-          leafOf[tr] = next;
-          posOf[tr] = p++;
-        }
-        next++;
-      } else {
-        stk.push_back(t.nodes[n].right);
-        stk.push_back(t.nodes[n].left);
-      }
-    }
-  }
+  const std::vector<int>& leafOf = t.tri_leaf;
+  const std::vector<int>& posOf = t.tri_pos;
   bool has = false;
   float best = -1;
   unsigned bt = 0;

@@ -77,6 +77,9 @@ struct Tree {
   std::vector<uint32_t> tri_v;   // 3 per triangle (KDTree::triangleList[i].vertexIndices)
   std::vector<Node> nodes;       // nodes[0] = root
   int abs_mode = ABS_DOUBLE;
+  // filled by build_tree: DFS (pre-order) leaf index per node (-1 interior),
+  // and per triangle its leaf index and position inside the leaf
+  std::vector<int> leaf_rank, tri_leaf, tri_pos;
 };
 
 void build_tree(Tree& t);  // KDTree::build(BEST) src/KDTree.cpp:230-246

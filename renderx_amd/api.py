@@ -21,9 +21,11 @@ ORDERED, EXHAUSTIVE, LEGALITY = 0, 1, 2
 
 RENDER_PARAMS = np.dtype([("spp", "<u4"), ("max_depth", "<u4"), ("rr_start_depth", "<u4"), ("mode", "<u4"),
                           ("seed", "<u8"), ("sample_offset", "<u4"), ("flags", "<u4")])
-RENDER_STATS = np.dtype([("rays", "<u8"), ("segments", "<u8"), ("samples", "<u8"), ("ms_total", "<f8"),
+RENDER_STATS = np.dtype([("rays", "<u8"), ("extend_rays", "<u8"), ("shadow_rays", "<u8"), ("segments", "<u8"),
+                         ("samples", "<u8"), ("box_tests", "<u8"), ("tri_tests", "<u8"), ("ms_total", "<f8"),
                          ("ms_extend", "<f8"), ("ms_shade", "<f8"), ("ms_shadow", "<f8"),
                          ("extend_launches", "<u8")])
+RENDER_TIMING, RENDER_COUNT_WORK, RENDER_NO_GRAPH = 1, 2, 4
 GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("samples_per_pass", "<u4"), ("learning_rate", "<f4"),
                          ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4")])
 MODE_PT, MODE_NEE, MODE_GUIDED = 0, 1, 2

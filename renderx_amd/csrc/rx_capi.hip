@@ -222,6 +222,8 @@ int rx_load_tree(rx_ctx* c, uint32_t n_nodes, const float* nodes4, const float* 
   if (!parse_serialized(n_nodes, nodes4, min4, max4, n_leaf_tris, v1, v2, v3, c->host_tree, c->leaf_verts))
     return set_err(c, RX_ERR_INVALID, "rx_load_tree: malformed serialized tree");
   c->scene_loaded = false;
+  c->tri_obj.clear();  // legality data belongs to the previous tree
+  c->tri_normal.clear();
   return upload_tree(c);
 }
 
