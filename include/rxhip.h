@@ -194,8 +194,29 @@ typedef struct rx_guide_params {
   uint32_t flags;
 } rx_guide_params;
 
+/* The guiding network has no counterpart in the reference.  It maps a
+ * frequency encoding of (position normalised to the scene box: 12 sin/cos
+ * octaves; view direction: 4 octaves) = 96 inputs through 3 ReLU layers of 64
+ * to 48 outputs (41 used): per lobe j of 8 a weight logit, an unnormalised
+ * mean direction and a log concentration, then one selection logit.
+ * Parameters are FP32 masters (RX_GUIDE_PARAMS floats: W1 96x64, W2 64x64,
+ * W3 64x64, W4 64x48 row-major in x out, then b1, b2, b3, b4); the kernels
+ * compute in BF16 on MFMA with FP32 accumulation. */
+#define RX_GUIDE_PARAMS 17648
+
 /* Configures and (re)initialises the guiding network used by RX_MODE_GUIDED. */
 int rx_guide_init(rx_ctx* ctx, const rx_guide_params* params);
+/* Box the position encoding normalises to (rx_load_scene sets the scene box). */
+int rx_guide_set_bounds(rx_ctx* ctx, const float* lo3, const float* hi3);
+int rx_guide_set_weights(rx_ctx* ctx, const float* params);
+int rx_guide_get_weights(rx_ctx* ctx, float* params);
+/* Network outputs (48 floats per query) for n positions/directions (3n floats each). */
+int rx_guide_eval(rx_ctx* ctx, uint32_t n, const float* pos, const float* dir, float* out48);
+/* One Adam step on n radiance samples: sampled direction omega (3n), BSDF
+ * density at omega, weight F/p_sample; loss = mean of -weight log(alpha q +
+ * (1-alpha) p_bsdf).  Optional outputs: mean loss, mean gradient (RX_GUIDE_PARAMS). */
+int rx_guide_train_batch(rx_ctx* ctx, uint32_t n, const float* pos, const float* dir, const float* omega,
+                         const float* pdf_bsdf, const float* weight, float* loss, float* grad);
 
 #ifdef __cplusplus
 }

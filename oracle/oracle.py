@@ -46,9 +46,10 @@ def lib():
         L.orc_scene_create.restype = P
         L.orc_scene_create.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P, P, C.c_int]
         L.orc_scene_free.argtypes = [P]
+        L.orc_scene_set_camera.argtypes = [P, P]
         L.orc_scene_tree.restype = P
         L.orc_scene_tree.argtypes = [P]
-        L.orc_render.argtypes = [P, P, P, P, C.c_int]
+        L.orc_render.argtypes = [P, P, P, P, C.c_int, P]
         L.orc_mt_nth.restype = C.c_uint32
         L.orc_mt_nth.argtypes = [C.c_uint32]
         L.orc_max_threads.restype = C.c_int
@@ -142,11 +143,22 @@ class SceneOracle:
             lib().orc_scene_free(self.h)
             self.h = None
 
+    def set_camera(self, camera):
+        """Render the same built scene through another camera (e.g. a smaller image)."""
+        self._c = np.ascontiguousarray(camera.reshape(1))
+        lib().orc_scene_set_camera(self.h, _p(self._c))
+        import copy
+
+        self.scene = copy.copy(self.scene)
+        self.scene.camera = camera
+
     def tree(self) -> Tree:
         return Tree(self._v, self._t, _handle=lib().orc_scene_tree(self.h), _owner=self)
 
-    def render(self, spp, max_depth=None, rr_start_depth=None, mode=0, seed=1, sample_offset=0, threads=0):
-        """FP64 per-pixel (sum, sum of squares), shape (H, W, 3)."""
+    def render(self, spp, max_depth=None, rr_start_depth=None, mode=0, seed=1, sample_offset=0, threads=0,
+               stats=None):
+        """FP64 per-pixel (sum, sum of squares), shape (H, W, 3).  If `stats` is a
+        dict it receives the number of traversal queries under "rays"."""
         cam = self.scene.camera
         H, W = int(cam["height"]), int(cam["width"])
         p = np.zeros((), RENDER_PARAMS)
@@ -159,7 +171,10 @@ class SceneOracle:
         p = p.reshape(1)
         s = np.zeros((H, W, 3), np.float64)
         q = np.zeros((H, W, 3), np.float64)
-        lib().orc_render(self.h, _p(p), _p(s), _p(q), threads)
+        rays = np.zeros(1, np.uint64)
+        lib().orc_render(self.h, _p(p), _p(s), _p(q), threads, _p(rays))
+        if stats is not None:
+            stats["rays"] = int(rays[0])
         return s, q
 
 
@@ -172,3 +187,80 @@ def sampler_draws(kind: int, n: int, param: float, param2: float = 0.0, seed: in
     out = np.empty((n, 3), np.float32)
     lib().orc_sampler_draws(kind, n, param, param2, seed, _p(out))
     return out
+
+
+# ---------------------------------------------------------------- guiding MLP (FP64)
+MLP_IN, MLP_H, MLP_OUT = 96, 64, 48
+MLP_SHAPES = [(96, 64), (64, 64), (64, 64), (64, 48)]
+MLP_NW = sum(a * b for a, b in MLP_SHAPES)
+MLP_NB = 64 * 3 + 48
+
+
+def _mlp_sig():
+    L = lib()
+    if not getattr(L, "_mlp_ready", False):
+        P = C.c_void_p
+        L.orc_mlp_encode.argtypes = [C.c_int, P, P, P, P, P]
+        L.orc_mlp_forward.argtypes = [C.c_int, P, P, P, P, P]
+        L.orc_mlp_loss.argtypes = [C.c_int, P, P, P, P, P, P, P]
+        L.orc_mlp_loss.restype = C.c_double
+        L.orc_mlp_backward.argtypes = [C.c_int, P, P, P, P, P, P]
+        L.orc_adam.argtypes = [C.c_int, P, P, P, P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
+        L._mlp_ready = True
+    return L
+
+
+def mlp_encode(pos, dirs, lo, hi):
+    L = _mlp_sig()
+    pos = np.ascontiguousarray(pos, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    out = np.empty((len(pos), MLP_IN), np.float64)
+    L.orc_mlp_encode(len(pos), _p(pos), _p(dirs), _p(lo), _p(hi), _p(out))
+    return out
+
+
+def mlp_forward(x, params):
+    """params: flat float64 [W1,W2,W3,W4 row-major (in x out), b1..b4]."""
+    L = _mlp_sig()
+    x = np.ascontiguousarray(x, np.float64)
+    W = np.ascontiguousarray(params[:MLP_NW], np.float64)
+    b = np.ascontiguousarray(params[MLP_NW:], np.float64)
+    z4 = np.empty((len(x), MLP_OUT), np.float64)
+    acts = np.empty((len(x), 3, MLP_H), np.float64)
+    L.orc_mlp_forward(len(x), _p(x), _p(W), _p(b), _p(z4), _p(acts))
+    return z4, acts
+
+
+def mlp_loss(z4, omega, pb, wt, want_grad=True):
+    L = _mlp_sig()
+    z4 = np.ascontiguousarray(z4, np.float64)
+    omega = np.ascontiguousarray(omega, np.float32)
+    pb = np.ascontiguousarray(pb, np.float32)
+    wt = np.ascontiguousarray(wt, np.float32)
+    g = np.empty_like(z4) if want_grad else None
+    q = np.empty(len(z4), np.float64)
+    a = np.empty(len(z4), np.float64)
+    tot = L.orc_mlp_loss(len(z4), _p(z4), _p(omega), _p(pb), _p(wt), _p(g), _p(q), _p(a))
+    return tot, g, q, a
+
+
+def mlp_backward(x, params, acts, dz4):
+    L = _mlp_sig()
+    x = np.ascontiguousarray(x, np.float64)
+    W = np.ascontiguousarray(params[:MLP_NW], np.float64)
+    acts = np.ascontiguousarray(acts, np.float64)
+    dz4 = np.ascontiguousarray(dz4, np.float64)
+    dW = np.empty(MLP_NW, np.float64)
+    db = np.empty(MLP_NB, np.float64)
+    L.orc_mlp_backward(len(x), _p(x), _p(W), _p(acts), _p(dz4), _p(dW), _p(db))
+    return np.concatenate([dW, db])
+
+
+def adam(p, m, v, g, lr, b1, b2, eps, t):
+    L = _mlp_sig()
+    p, m, v = [np.ascontiguousarray(a, np.float64).copy() for a in (p, m, v)]
+    g = np.ascontiguousarray(g, np.float64)
+    L.orc_adam(len(p), _p(p), _p(m), _p(v), _p(g), lr, b1, b2, eps, t)
+    return p, m, v

@@ -208,11 +208,35 @@ void* orc_scene_create(int nv, const float* v, int nt, const uint32_t* tri, int 
 
 void orc_scene_free(void* h) { delete (SceneO*)h; }
 
+// Replace the camera (and its per-pixel vignetting factors) of a built scene.
+void orc_scene_set_camera(void* h, const void* cam) {
+  SceneO& sc = *(SceneO*)h;
+  const CamDescC* cd = (const CamDescC*)cam;
+  CamO& C = sc.cam;
+  C.position = V3(cd->position[0], cd->position[1], cd->position[2]);
+  C.focus = V3(cd->focus[0], cd->focus[1], cd->focus[2]);
+  C.up = V3(cd->up[0], cd->up[1], cd->up[2]);
+  C.sightDist = cd->sight_dist;
+  C.width = cd->width;
+  C.height = cd->height;
+  sc.cam_front = normalize(C.focus - C.position);
+  sc.cam_right = normalize(cross(sc.cam_front, C.up));
+  sc.cam_up = cross(sc.cam_right, sc.cam_front);
+  sc.vignette.resize((size_t)C.width * C.height);
+  for (int y = 0; y < C.height; y++)
+    for (int x = 0; x < C.width; x++) {
+      V3 dir = sc.cam_front * (float)C.sightDist + sc.cam_right * (float)(x + 0.5 - (C.width / 2.0)) +
+               sc.cam_up * (float)((C.height / 2.0) - 0.5 - y);
+      dir = normalize(dir);
+      sc.vignette[(size_t)y * C.width + x] = std::fabs(dot(dir, sc.cam_front));
+    }
+}
+
 void* orc_scene_tree(void* h) { return &((SceneO*)h)->tree; }
 
-void orc_render(void* h, const void* params, double* sum, double* sumsq, int threads) {
+void orc_render(void* h, const void* params, double* sum, double* sumsq, int threads, uint64_t* rays) {
   if (threads > 0) omp_set_num_threads(threads);
-  render(*(SceneO*)h, *(const RenderParamsO*)params, sum, sumsq, nullptr);
+  render(*(SceneO*)h, *(const RenderParamsO*)params, sum, sumsq, rays);
 }
 
 uint32_t orc_mt_nth(uint32_t n) { return mt_nth(n); }

@@ -26,6 +26,8 @@
 namespace orc {
 
 static const float PI_F = (float)M_PI;
+// traversal queries issued by this thread (camera, bounce, shadow, containment)
+static thread_local uint64_t g_rays = 0;
 static const float EPS = 1e-5f;          // EPSILON (include/macros.h:7)
 static const float COS_MIN = 1e-6f;      // COS_TERM_MIN (include/macros.h:9)
 
@@ -190,6 +192,7 @@ static float scene_intersect(const SceneO& sc, const PRay& r, bool useCond, int&
   Ray kr{r.origin, r.direction};
   Legality L{sc.tri_obj.data(), sc.tri_normal.data(), r.insideObject, r.contactObject};
   unsigned tid = 0;
+  g_rays++;
   float d = intersect_exhaustive(sc.tree, kr, tid, useCond ? &L : nullptr, nullptr);
   if (d >= 0) {
     obj = sc.tri_obj[tid];
@@ -205,9 +208,11 @@ static int find_inside(const SceneO& sc, V3 o, V3 d, int current) {
     if (current == i || !is_volumetric(sc, i)) continue;
     const Tree& t = sc.obj_tree[i];
     unsigned tid = 0;
+    g_rays++;
     float dist = intersect_exhaustive(t, back, tid, nullptr, nullptr);
     if (dist < 1e-6f) continue;
     if (dot(sc.tri_normal[sc.objs[i].tri_begin + tid], back.d) < 1e-6f) continue;
+    g_rays++;
     dist = intersect_exhaustive(t, front, tid, nullptr, nullptr);
     if (dist < 1e-6f) continue;
     if (dot(sc.tri_normal[sc.objs[i].tri_begin + tid], front.d) < 1e-6f) continue;
@@ -839,8 +844,10 @@ static V3 pttest_sample(Ctx& c, unsigned pixel, std::vector<PRay>& path) {
 
 void render(const SceneO& sc, const RenderParamsO& p, double* sum, double* sumsq, uint64_t* rays) {
   const unsigned npix = (unsigned)(sc.cam.width * sc.cam.height);
-#pragma omp parallel for schedule(static)
+  uint64_t total_rays = 0;
+#pragma omp parallel for schedule(static) reduction(+ : total_rays)
   for (int px = 0; px < (int)npix; px++) {
+    g_rays = 0;
     std::vector<PRay> path;
     double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
     for (unsigned smp = 0; smp < p.spp; smp++) {
@@ -856,8 +863,9 @@ void render(const SceneO& sc, const RenderParamsO& p, double* sum, double* sumsq
       sum[3 * px + k] += s[k];
       if (sumsq) sumsq[3 * px + k] += q[k];
     }
+    total_rays += g_rays;
   }
-  (void)rays;
+  if (rays) *rays += total_rays;
 }
 
 // Draws from the restated samplers, for distribution tests:

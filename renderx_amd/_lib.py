@@ -35,7 +35,13 @@ SIGNATURES = {
     "rx_load_scene": (C.c_int, [P, U32, P, U32, P, U32, P, U32]),
     "rx_render": (C.c_int, [P, P, P, P, P, P, P]),
     "rx_guide_init": (C.c_int, [P, P]),
+    "rx_guide_set_bounds": (C.c_int, [P, P, P]),
+    "rx_guide_set_weights": (C.c_int, [P, P]),
+    "rx_guide_get_weights": (C.c_int, [P, P]),
+    "rx_guide_eval": (C.c_int, [P, U32, P, P, P]),
+    "rx_guide_train_batch": (C.c_int, [P, U32, P, P, P, P, P, P, P]),
 }
+GUIDE_PARAMS = 17648
 
 _lib = None
 

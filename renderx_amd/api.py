@@ -144,7 +144,7 @@ class PathTracerHIP:
         self.ctx.check(lib().rx_guide_init(self.ctx.h, ptr(p)), "rx_guide_init")
 
     def renderPixels(self, camera=None, spp=None, max_depth=None, rr_start_depth=None, mode=MODE_PT, seed=1,
-                     sample_offset=0, want_sums=True):
+                     sample_offset=0, want_sums=True, flags=0):
         """Returns dict(mean=(H,W,3) f32, sum, sumsq=(H,W,3) f64, stats)."""
         cam = np.ascontiguousarray((self.scene.camera if camera is None else camera).reshape(1))
         H, W = int(cam["height"][0]), int(cam["width"][0])
@@ -155,6 +155,7 @@ class PathTracerHIP:
         p["mode"] = mode
         p["seed"] = seed
         p["sample_offset"] = sample_offset
+        p["flags"] = flags
         mean = np.empty((H, W, 3), np.float32)
         s = np.empty((H, W, 3), np.float64) if want_sums else None
         q = np.empty((H, W, 3), np.float64) if want_sums else None

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "rx_context.h"
+#include "rx_guide.h"
 #include "rx_kernels.h"
 
 namespace rx {
@@ -83,6 +84,20 @@ struct PathBufs {
 struct PassArgs {
   uint32_t n_paths, sample, key1, max_depth, rr_start, mode;
 };
+
+// Guided mode: network outputs per queue row of the current bounce and the
+// training records of the current pass (16 floats per (path, bounce) slot:
+// x, -d, w, p_mix(w), p_bsdf(w), lum L before the vertex, lum throughput
+// arriving at the vertex, RR probability, lum of the vertex's own NEE, tag).
+struct GuideArgs {
+  GuideNet net;
+  const float* Z4;
+  uint16_t* X;
+  float* rec;
+  uint32_t R;
+  float tag;  // > 0: record this pass with that tag
+};
+constexpr int kRecFloats = 16;
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ float lum(Vec3 v) { return 0.212671f * v.x + 0.715160f * v.y + 0.072169f * v.z; }
@@ -280,6 +295,22 @@ __global__ void __launch_bounds__(kRBlock) k_extend(DevTree T, PathBufs B, uint3
   }
 }
 
+// ------------------------------------------------------------------ guide queries
+// Encode (hit point, view direction) of every queued path of bounce b; row i
+// of the network batch belongs to queue entry i.
+__global__ void __launch_bounds__(kRBlock) k_guide_query(PathBufs B, GuideArgs GA, uint32_t b) {
+  const uint32_t n = B.cnt[b];
+  const uint32_t* q = (b & 1) ? B.q1 : B.q0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t p = q[i];
+    const float4 o = B.o[p], d = B.d[p];
+    const int2 h = B.hit[p];
+    const float t = h.y >= 0 ? __int_as_float(h.x) : 0.0f;
+    Vec3 dd = xyz(d);
+    guide_encode(GA.net, xyz(o) + dd * t, -dd, GA.X + (size_t)i * kGIn);
+  }
+}
+
 // ------------------------------------------------------------------ shade
 struct Scat {
   Vec3 x, dir, color;
@@ -289,7 +320,83 @@ struct Scat {
   int neeKind;   // 0 none, 1 surface (neeObj), 2 medium (neeObj)
   int neeObj;
   float pdfB;    // density of the continuation direction (RR survival included)
+  bool guided;   // sampled by the guide/BSDF one-sample MIS mixture
+  float pmix, pb, qrr;
 };
+
+// Effective BSDF f (Phong: half the lobe sum, the expectation of the uniform
+// lobe pick) and the BSDF-sampling density without Russian roulette.
+__device__ __forceinline__ void bsdf_nrr(const DevObject& O, Vec3 n, Vec3 d, Vec3 w, Vec3& f, float& pdf) {
+  const bool front = dot(n, d) <= 0 && dot(n, w) >= 0;
+  Vec3 fd = v3(0, 0, 0), fg = v3(0, 0, 0);
+  float pd = 0.0f, pg = 0.0f;
+  if (O.type == OBJ_DIFFUSE || O.type == OBJ_PHONG) {
+    Vec3 rho = O.type == OBJ_DIFFUSE ? ld3(O.c1) : ld3(O.c2);
+    if (front) fd = rho * 0.31830988618379067154f;
+    pd = fmaxf(dot(n, w), 0.0f) * 0.31830988618379067154f;
+  }
+  if (O.type == OBJ_GLOSSY || O.type == OBJ_PHONG) {
+    Vec3 r = normalize(mirror(n, d));
+    float rw = dot(r, w);
+    if (front && rw > 0) fg = ld3(O.c1) * powf(rw, O.expo);
+    pg = lobe_pdf(r, w, O.expo);
+  }
+  if (O.type == OBJ_PHONG) {
+    f = (fd + fg) * 0.5f;
+    pdf = 0.5f * (pd + pg);
+  } else {
+    f = fd + fg;
+    pdf = pd + pg;
+  }
+}
+__device__ __forceinline__ float guided_rr(const DevObject& O) {
+  if (O.type == OBJ_PHONG)
+    return fminf(1.0f, fmaxf(O.c1[0] + O.c2[0], fmaxf(O.c1[1] + O.c2[1], O.c1[2] + O.c2[2])));
+  return O.max1;
+}
+
+// Guided surface scatter: with probability alpha sample the learned vMF
+// mixture, otherwise the reference's BSDF strategy; weight f cos / p_mix with
+// p_mix = alpha q + (1 - alpha) p_bsdf (one-sample MIS, unbiased because
+// alpha <= 0.95 keeps the BSDF density in the mixture).
+__device__ void guided_scatter(const DevObject& O, int obj, Vec3 x, Vec3 d, Vec3 n, int hitPos, int insideIn,
+                               bool russian, const GuideHead& H, const float* ug, float u_rr, Scat& sc) {
+  sc.x = x;
+  sc.contact = obj;
+  sc.contactPos = hitPos;
+  sc.inside = insideIn;
+  sc.random = true;
+  sc.neeKind = 1;
+  sc.neeObj = obj;
+  sc.guided = true;
+  Vec3 w;
+  if (ug[0] < H.alpha) {
+    w = guide_sample(H, ug[1], ug[2], ug[3]);
+  } else {
+    bool glossy = O.type == OBJ_GLOSSY || (O.type == OBJ_PHONG && ug[1] < 0.5f);
+    if (glossy) w = sample_lobe(frame_from(normalize(mirror(n, d))), ug[2], ug[3], O.expo);
+    else w = sample_lobe(frame_from(n), ug[2], ug[3], 1.0f);
+  }
+  Vec3 f;
+  float pb;
+  bsdf_nrr(O, n, d, w, f, pb);
+  const float q = guide_pdf(H, w);
+  const float pmix = H.alpha * q + (1.0f - H.alpha) * pb;
+  const float qrr = russian ? guided_rr(O) : 1.0f;
+  sc.pmix = pmix;
+  sc.pb = pb;
+  sc.qrr = qrr;
+  if (maxc(f) <= 0.0f || !(pmix > 0.0f) || u_rr >= qrr) {
+    sc.dir = v3(0, 0, 0);
+    sc.color = v3(0, 0, 0);
+    sc.dirProb = 1.0f;
+    return;
+  }
+  sc.dir = w;
+  sc.color = f;
+  sc.dirProb = pmix * qrr;
+  sc.pdfB = pmix * qrr;
+}
 
 // Density of the continuation direction w at a surface vertex of object O with
 // normal n and incoming direction d (Phong: mixture of its two lobes).
@@ -419,7 +526,7 @@ __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassArgs P, uint32_t b) {
+__global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
   const uint32_t n = B.cnt[b];
   const uint32_t* qin = (b & 1) ? B.q1 : B.q0;
@@ -465,6 +572,10 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
         sc.random = sc.emitter = sc.transparent = false;
         sc.neeKind = 0;
         sc.neeObj = NONE;
+        sc.pdfB = 0.0f;
+        sc.guided = false;
+        sc.pmix = sc.pb = sc.qrr = 0.0f;
+        GuideHead head;
         const Vec3 xh = o + d * t;  // hit point (valid when pos >= 0)
         int cur = obj;
         float originScale = 1.0f;
@@ -558,6 +669,14 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
             if (pos < 0) {
               sc.x = o;
               done = true;
+            } else if (MODE == 2 && dot(d, hn) <= 0 &&
+                       (S.objs[cur].type == OBJ_DIFFUSE || S.objs[cur].type == OBJ_GLOSSY ||
+                        S.objs[cur].type == OBJ_PHONG)) {
+              guide_head(GA.Z4 + (size_t)i * kGOut, head);
+              U4 r3 = philox(U4{P.sample, b, 3u, 0u}, p, P.key1);
+              const float ug[4] = {u01(r3.x), u01(r3.y), u01(r3.z), u01(r3.w)};
+              guided_scatter(S.objs[cur], cur, xh, d, hn, pos, inside, russian, head, ug, u[2], sc);
+              done = true;
             } else {
               surface_scatter(S, cur, xh, d, hn, pos, inside, russian, u, sc, lds, B, rays);
               done = true;
@@ -592,7 +711,12 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
             float pL = r2l / (cosL * S.total_area);
             Vec3 f;
             float pB, cx = 1.0f;
-            if (sc.neeKind == 1) {
+            if (sc.neeKind == 1 && sc.guided) {
+              float pbl;
+              bsdf_nrr(S.objs[sc.neeObj], hn, d, w, f, pbl);
+              pB = sc.qrr * (head.alpha * guide_pdf(head, w) + (1.0f - head.alpha) * pbl);
+              cx = fmaxf(fabsf(dot(hn, w)), 1e-6f);
+            } else if (sc.neeKind == 1) {
               surface_bsdf(S.objs[sc.neeObj], hn, d, w, russian, f, pB);
               cx = fmaxf(fabsf(dot(hn, w)), 1e-6f);
             } else {
@@ -636,14 +760,23 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
             float cosT = 1.0f;
             if (sc.contact != NONE && S.objs[sc.contact].type != OBJ_VPM)
               cosT = fmaxf(fabsf(dot(nc, sc.dir)), 1e-6f);
+            const Vec3 beta_arr = beta;  // throughput arriving at the vertex
             beta = beta * sc.color * (cosT / sc.dirProb);
             uint32_t flags = (uint32_t)s.w;
             float lastPdf = be.w;
             if (sc.random) {
               flags = FLAG_MIS;
-              float f_unused;
-              (void)f_unused;
-              if (sc.neeKind == 1) {
+              if (sc.guided) {
+                lastPdf = sc.pdfB;
+                if (GA.tag > 0.0f) {  // training record of this vertex
+                  float* r = GA.rec + ((size_t)p * GA.R + b) * kRecFloats;
+                  const float4 Lp = B.L[p];
+                  const float vals[kRecFloats] = {sc.x.x, sc.x.y, sc.x.z, -d.x, -d.y, -d.z, sc.dir.x, sc.dir.y,
+                                                  sc.dir.z, sc.pmix, sc.pb, lum(v3(Lp.x, Lp.y, Lp.z)), lum(beta_arr),
+                                                  sc.qrr, 0.0f, GA.tag};
+                  for (int k = 0; k < kRecFloats; k++) r[k] = vals[k];
+                }
+              } else if (sc.neeKind == 1) {
                 Vec3 fdummy;
                 surface_bsdf(S.objs[sc.neeObj], hn, d, sc.dir, russian, fdummy, lastPdf);
               } else {
@@ -678,7 +811,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
 // visible when the first legal hit is not nearer than the light point
 // (minus EPSILON).  Index-matched medium boundaries are crossed with the
 // transmittance the PathTracer's distance sampling realises on average.
-__global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S, PathBufs B, uint32_t b) {
+__global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S, PathBufs B, uint32_t b, GuideArgs GA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
   const uint32_t n = B.cnt[kShadowBase + b];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -728,12 +861,101 @@ __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S, PathBufs B, uint
     }
     if (visible) {
       float4 L = B.L[p];
-      B.L[p] = make_float4(L.x + scv.x * T.x, L.y + scv.y * T.y, L.z + scv.z * T.z, 0.0f);
+      const Vec3 add = v3(scv.x * T.x, scv.y * T.y, scv.z * T.z);
+      B.L[p] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.0f);
+      if (GA.tag > 0.0f) {  // the vertex's own NEE is not part of the radiance along its sampled direction
+        float* r = GA.rec + ((size_t)p * GA.R + b) * kRecFloats;
+        if (r[15] == GA.tag) r[14] += lum(add);
+      }
     }
   }
   if (ovf) *B.overflow = 1;
   wave_add(&B.stats[0], rays);
   wave_add(&B.stats[2], srays);
+}
+
+// ------------------------------------------------------------------ training samples
+// Deterministic selection of the pass's guided-vertex records: records are
+// ranked in (path, bounce) order by a chunked count / scan / write, and an
+// evenly strided subset of at most `cap` becomes the training batch with
+// weight F/p_sample = lum(L after the vertex) * q_rr / lum(throughput at it).
+constexpr int kRecChunk = 1024;
+
+__global__ void k_rec_count(const float* rec, uint32_t K, float tag, uint32_t* bcount) {
+  __shared__ uint32_t red[256];
+  uint32_t c = 0;
+  const uint32_t base = blockIdx.x * kRecChunk + threadIdx.x * 4;
+  for (int j = 0; j < 4; j++) {
+    const uint32_t k = base + j;
+    if (k < K && rec[(size_t)k * kRecFloats + 15] == tag) c++;
+  }
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bcount[blockIdx.x] = red[0];
+}
+
+__global__ void k_rec_scan(uint32_t* bcount, uint32_t nb, uint32_t* total) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t s = 0;
+  for (uint32_t i = 0; i < nb; i++) {
+    const uint32_t c = bcount[i];
+    bcount[i] = s;
+    s += c;
+  }
+  *total = s;
+}
+
+struct TrainOut {
+  float4 *pos, *dir, *wdir;
+  float *pb, *wt;
+  uint32_t* rows;
+};
+
+__global__ void k_rec_write(const float* rec, uint32_t K, float tag, const uint32_t* boff, const uint32_t* total,
+                            uint32_t cap, const float4* L, uint32_t R, TrainOut T) {
+  __shared__ uint32_t sc[256];
+  uint32_t flags[4];
+  uint32_t c = 0;
+  const uint32_t base = blockIdx.x * kRecChunk + threadIdx.x * 4;
+  for (int j = 0; j < 4; j++) {
+    const uint32_t k = base + j;
+    flags[j] = (k < K && rec[(size_t)k * kRecFloats + 15] == tag) ? 1u : 0u;
+    c += flags[j];
+  }
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {  // inclusive scan (Hillis-Steele)
+    uint32_t v = threadIdx.x >= (uint32_t)o ? sc[threadIdx.x - o] : 0u;
+    __syncthreads();
+    sc[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const uint32_t Ttot = *total;
+  const uint32_t Bsel = min(Ttot, cap);
+  uint32_t g = boff[blockIdx.x] + sc[threadIdx.x] - c;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *T.rows = Bsel;
+  for (int j = 0; j < 4; j++) {
+    if (!flags[j]) continue;
+    const uint32_t k = base + j;
+    const uint64_t lo = (uint64_t)g * Bsel / Ttot, hi = (uint64_t)(g + 1) * Bsel / Ttot;
+    g++;
+    if (hi == lo) continue;  // not in the strided subset
+    const uint32_t slot = (uint32_t)lo;
+    const float* r = rec + (size_t)k * kRecFloats;
+    const uint32_t p = k / R;
+    const float4 Lp = L[p];
+    const float dl = lum(v3(Lp.x, Lp.y, Lp.z)) - r[11] - r[14];
+    const float w = (r[12] > 1e-20f && dl > 0.0f) ? dl * r[13] / r[12] : 0.0f;
+    T.pos[slot] = make_float4(r[0], r[1], r[2], 0.0f);
+    T.dir[slot] = make_float4(r[3], r[4], r[5], 0.0f);
+    T.wdir[slot] = make_float4(r[6], r[7], r[8], 0.0f);
+    T.pb[slot] = r[10];
+    T.wt[slot] = isfinite(w) ? w : 0.0f;
+  }
 }
 
 // ------------------------------------------------------------------ film
@@ -777,7 +999,9 @@ struct SceneBufs {
 struct RenderState {
   SceneBufs scene;
   DBuf o, d, beta, lastp, L, s, hit, q0, q1, sq, cnt, so, sd, sc, ss, sum, sumsq, mean, stats;
+  DBuf rec, bcount, total;  // guided-mode training records
   uint32_t cap = 0;
+  float box_lo[3], box_hi[3];
 };
 
 static void leaf_verts_of(const HostTree& t, const float* verts, const uint32_t* tris, std::vector<float>& lv) {
@@ -839,6 +1063,8 @@ int rx_load_scene(rx_ctx* c, uint32_t n_verts, const float* verts, uint32_t n_tr
   }
   SceneBufs& SB = R->scene;
   SB.max_depth = c->host_tree.max_depth;
+  memcpy(R->box_lo, c->host_tree.nodes[0].lo, 12);
+  memcpy(R->box_hi, c->host_tree.nodes[0].hi, 12);
   // objects
   std::vector<DevTree> vt;
   std::vector<int32_t> vobj;
@@ -1029,7 +1255,36 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   const DevScene S = R.scene.dev;
   const bool timing = (prm->flags & RX_RENDER_TIMING) != 0;
   const bool count = (prm->flags & RX_RENDER_COUNT_WORK) != 0;
-  const int mode = prm->mode >= RX_MODE_NEE ? 1 : 0;
+  const int mode = (int)prm->mode;
+  GuideArgs GA{};
+  uint32_t train_passes = 0, train_cap = 0;
+  if (mode == RX_MODE_GUIDED) {
+    if (!c->guide) return set_err(c, RX_ERR_STATE, "rx_render: guided mode needs rx_guide_init");
+    GuideState& G = *c->guide;
+    for (int a = 0; a < 3; a++) {  // encode positions relative to the scene box (slightly padded)
+      float pad = 1e-3f * (R.box_hi[a] - R.box_lo[a]) + 1e-6f;
+      G.lo[a] = R.box_lo[a] - pad;
+      G.hi[a] = R.box_hi[a] + pad;
+    }
+    // samples with index < train_passes train the network online (a render
+    // resumed with sample_offset >= train_passes only uses it)
+    train_passes = G.prm.train_passes;
+    train_cap = std::max(1024u, G.prm.samples_per_pass);
+    if (int r = guide_reserve(c, G, std::max(N, train_cap))) return r;
+    const uint32_t Rs = prm->max_depth + 1;
+    if (prm->sample_offset < train_passes && prm->spp) {
+      RXCK(R.rec.ensure((size_t)N * Rs * kRecFloats * 4), "alloc records");
+      RXCK(R.bcount.ensure(((size_t)N * Rs / kRecChunk + 2) * 4), "alloc records");
+      RXCK(R.total.ensure(64), "alloc records");
+      // tags restart at 1 every render: clear stale records once
+      hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, R.rec.as<double>(), (size_t)N * Rs * kRecFloats / 2);
+    }
+    GA.net = G.net();
+    GA.Z4 = G.Z4.as<float>();
+    GA.X = G.X.as<uint16_t>();
+    GA.rec = R.rec.as<float>();
+    GA.R = Rs;
+  }
   std::vector<hipEvent_t> evs;
   auto mark = [&](void) -> hipEvent_t {
     hipEvent_t e;
@@ -1047,6 +1302,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   uint64_t launches = 0;
   for (uint32_t pass = 0; pass < prm->spp; pass++) {
     P.sample = prm->sample_offset + pass;
+    GA.tag = (mode == RX_MODE_GUIDED && P.sample < train_passes) ? (float)(pass + 1) : 0.0f;
     hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P);
     for (uint32_t b = 0; b <= prm->max_depth; b++) {
       hipEvent_t e0 = timing ? mark() : nullptr;
@@ -1059,12 +1315,20 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       }
       launches++;
       hipEvent_t e1 = timing ? mark() : nullptr;
-      if (mode) hipLaunchKernelGGL((k_shade<1>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b);
-      else hipLaunchKernelGGL((k_shade<0>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b);
+      if (mode == RX_MODE_GUIDED) {  // network inference for this bounce's hit points
+        hipLaunchKernelGGL(k_guide_query, dim3(blocks), dim3(kRBlock), 0, s, B, GA, b);
+        RXCK(guide_forward(GA.net, GA.X, B.cnt + b, N, nullptr, nullptr, nullptr, c->guide->Z4.as<float>(), c->n_cu, s),
+             "guide forward");
+        hipLaunchKernelGGL((k_shade<2>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA);
+      } else if (mode == RX_MODE_NEE) {
+        hipLaunchKernelGGL((k_shade<1>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA);
+      } else {
+        hipLaunchKernelGGL((k_shade<0>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA);
+      }
       hipEvent_t e2 = timing ? mark() : nullptr;
       hipEvent_t e3 = nullptr;
       if (mode) {
-        hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(kRBlock), 0, s, S, B, b);
+        hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(kRBlock), 0, s, S, B, b, GA);
         e3 = timing ? mark() : nullptr;
       }
       if (timing) {
@@ -1075,6 +1339,19 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     }
     hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(kRBlock), 0, s, B, N);
     RXCK(hipGetLastError(), "launch pass");
+    if (GA.tag > 0.0f) {  // online training on this pass's radiance samples
+      GuideState& G = *c->guide;
+      const uint32_t K = N * GA.R;
+      const uint32_t nb = (K + kRecChunk - 1) / kRecChunk;
+      hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(256), 0, s, GA.rec, K, GA.tag, R.bcount.as<uint32_t>());
+      hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(64), 0, s, R.bcount.as<uint32_t>(), nb, R.total.as<uint32_t>());
+      TrainOut TO{G.pos.as<float4>(), G.dir.as<float4>(), G.wdir.as<float4>(), G.pb.as<float>(), G.wt.as<float>(),
+                  G.rows.as<uint32_t>()};
+      hipLaunchKernelGGL(k_rec_write, dim3(nb), dim3(256), 0, s, GA.rec, K, GA.tag, R.bcount.as<uint32_t>(),
+                         R.total.as<uint32_t>(), train_cap, B.L, GA.R, TO);
+      RXCK(hipGetLastError(), "launch gather");
+      if (int r = guide_train_rows(c, train_cap)) return r;
+    }
   }
   RXCK(hipEventRecord(c->ev1, s), "event");
   if (rgb_mean && prm->spp) {
