@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only"}))
@@ -149,6 +151,14 @@ def main():
                      "kernel": "k_extend (nearest-hit traversal)",
                      "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_launch_s * 1e3},
     }
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(scene)

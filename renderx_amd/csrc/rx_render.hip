@@ -83,6 +83,7 @@ struct PathBufs {
 
 struct PassArgs {
   uint32_t n_paths, sample, key1, max_depth, rr_start, mode;
+  uint32_t npix;  // pixels; path slot p renders pixel p % npix, sample `sample + p / npix`
 };
 
 // Guided mode: network outputs per queue row of the current bounce and the
@@ -241,8 +242,9 @@ __global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C, Path
     for (int k = threadIdx.x; k < kCntWords; k += blockDim.x) B.cnt[k] = (k == 0) ? P.n_paths : 0u;
   const Vec3 front = ld3(C.front), right = ld3(C.right), up = ld3(C.up), pos = ld3(C.pos);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n_paths; i += stride) {
-    const int x = (int)(i % (uint32_t)C.W), y = (int)(i / (uint32_t)C.W);
-    U4 r = philox(U4{P.sample, 0xFFFFu, 0u, 0u}, i, P.key1);
+    const uint32_t pix = i % P.npix, smp = P.sample + i / P.npix;
+    const int x = (int)(pix % (uint32_t)C.W), y = (int)(pix / (uint32_t)C.W);
+    U4 r = philox(U4{smp, 0xFFFFu, 0u, 0u}, pix, P.key1);
     float xp = ((float)x + u01(r.x)) - (float)C.W * 0.5f;
     float yp = (float)C.H * 0.5f - ((float)y + u01(r.y));
     Vec3 d = normalize(front * C.sight + right * xp + up * yp);
@@ -541,6 +543,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
     uint32_t p = 0;
     if (i < n) {
       p = qin[i];
+      const uint32_t pix = p % P.npix, smp = P.sample + p / P.npix;
       segs++;
       const float4 o4 = B.o[p], d4 = B.d[p], be = B.beta[p];
       const int4 s = B.s[p];
@@ -550,8 +553,8 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
       const int inside = s.x;
       const float t = __int_as_float(h.x);
       const int pos = h.y;
-      U4 r0 = philox(U4{P.sample, b, 0u, 0u}, p, P.key1);
-      U4 r1 = philox(U4{P.sample, b, 1u, 0u}, p, P.key1);
+      U4 r0 = philox(U4{smp, b, 0u, 0u}, pix, P.key1);
+      U4 r1 = philox(U4{smp, b, 1u, 0u}, pix, P.key1);
       const float u[8] = {u01(r0.x), u01(r0.y), u01(r0.z), u01(r0.w), u01(r1.x), u01(r1.y), u01(r1.z), u01(r1.w)};
       int hitObj = NONE;
       Vec3 hn = v3(0, 0, 0);
@@ -673,7 +676,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
                        (S.objs[cur].type == OBJ_DIFFUSE || S.objs[cur].type == OBJ_GLOSSY ||
                         S.objs[cur].type == OBJ_PHONG)) {
               guide_head(GA.Z4 + (size_t)i * kGOut, head);
-              U4 r3 = philox(U4{P.sample, b, 3u, 0u}, p, P.key1);
+              U4 r3 = philox(U4{smp, b, 3u, 0u}, pix, P.key1);
               const float ug[4] = {u01(r3.x), u01(r3.y), u01(r3.z), u01(r3.w)};
               guided_scatter(S.objs[cur], cur, xh, d, hn, pos, inside, russian, head, ug, u[2], sc);
               done = true;
@@ -687,10 +690,11 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
         // segment into the new vertex: decay by the incoming ray's medium, 1/originProb
         const float seg = length(sc.x - o);
         beta = beta * decay(S, inside, seg);
-        beta = beta * (1.0f / sc.originProb);
+        // vec3 / 0 leaves the vector unchanged in the reference (nvVector.h operator/=)
+        if (sc.originProb != 0.0f) beta = beta * (1.0f / sc.originProb);
         if (MODE >= 1 && sc.neeKind) {
           // next-event estimation to an area light: one light point, MIS with the continuation density
-          U4 r2 = philox(U4{P.sample, b, 2u, 0u}, p, P.key1);
+          U4 r2 = philox(U4{smp, b, 2u, 0u}, pix, P.key1);
           float us = u01(r2.x), u1 = u01(r2.y), u2 = u01(r2.z);
           int lo = 0, hi = S.n_etri - 1;
           while (lo < hi) {
@@ -736,7 +740,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
           }
         }
         if (sc.emitter) {
-          Vec3 le = sc.color * (1.0f / sc.dirProb);
+          Vec3 le = sc.dirProb != 0.0f ? sc.color * (1.0f / sc.dirProb) : sc.color;
           float w = 1.0f;
           if (MODE >= 1 && (s.w & FLAG_MIS)) {
             Vec3 lp = xyz(B.lastp[p]);
@@ -761,7 +765,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
             if (sc.contact != NONE && S.objs[sc.contact].type != OBJ_VPM)
               cosT = fmaxf(fabsf(dot(nc, sc.dir)), 1e-6f);
             const Vec3 beta_arr = beta;  // throughput arriving at the vertex
-            beta = beta * sc.color * (cosT / sc.dirProb);
+            beta = beta * sc.color * (sc.dirProb != 0.0f ? cosT / sc.dirProb : cosT);
             uint32_t flags = (uint32_t)s.w;
             float lastPdf = be.w;
             if (sc.random) {
@@ -959,14 +963,28 @@ __global__ void k_rec_write(const float* rec, uint32_t K, float tag, const uint3
 }
 
 // ------------------------------------------------------------------ film
-__global__ void k_accumulate(PathBufs B, uint32_t n) {
+// Adds the pass's samples of each pixel in sample order (slot j*npix + pixel
+// holds sample `sample + j`), so the FP64 film is independent of how samples
+// are grouped into passes.
+__global__ void k_accumulate(PathBufs B, uint32_t npix, uint32_t S) {
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
-    float4 L = B.L[p];
-    double l[3] = {L.x, L.y, L.z};
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < npix; q += stride) {
+    double s[3], s2[3];
     for (int k = 0; k < 3; k++) {
-      B.sum[3 * (size_t)p + k] += l[k];
-      B.sumsq[3 * (size_t)p + k] += l[k] * l[k];
+      s[k] = B.sum[3 * (size_t)q + k];
+      s2[k] = B.sumsq[3 * (size_t)q + k];
+    }
+    for (uint32_t j = 0; j < S; j++) {
+      const float4 L = B.L[(size_t)j * npix + q];
+      const double l[3] = {L.x, L.y, L.z};
+      for (int k = 0; k < 3; k++) {
+        s[k] += l[k];
+        s2[k] += l[k] * l[k];
+      }
+    }
+    for (int k = 0; k < 3; k++) {
+      B.sum[3 * (size_t)q + k] = s[k];
+      B.sumsq[3 * (size_t)q + k] = s2[k];
     }
   }
 }
@@ -1175,7 +1193,10 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   (void)hipSetDevice(c->device);
   RenderState& R = *c->render;
   hipStream_t s = c->stream;
-  const uint32_t N = (uint32_t)(cam->width * cam->height);
+  const uint32_t npix = (uint32_t)(cam->width * cam->height);
+  // samples per pass: enough path slots (~1M) to fill the GPU on small images
+  const uint32_t S_max = std::max<uint32_t>(1u, std::min<uint32_t>(std::max(prm->spp, 1u), (1u << 20) / npix));
+  const uint32_t N = npix * S_max;  // path slots
 #define RXCK(x, w) \
   if (int r_ = hip_err(c, (x), w)) return r_;
   if (R.cap < N) {
@@ -1193,11 +1214,11 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     RXCK(R.sd.ensure(16 * (size_t)N), "alloc");
     RXCK(R.sc.ensure(16 * (size_t)N), "alloc");
     RXCK(R.ss.ensure(8 * (size_t)N), "alloc");
-    RXCK(R.sum.ensure(24 * (size_t)N), "alloc");
-    RXCK(R.sumsq.ensure(24 * (size_t)N), "alloc");
-    RXCK(R.mean.ensure(12 * (size_t)N), "alloc");
     R.cap = N;
   }
+  RXCK(R.sum.ensure(24 * (size_t)npix), "alloc");
+  RXCK(R.sumsq.ensure(24 * (size_t)npix), "alloc");
+  RXCK(R.mean.ensure(12 * (size_t)npix), "alloc");
   RXCK(R.cnt.ensure(4 * kCntWords), "alloc");
   RXCK(R.stats.ensure(8 * 8), "alloc");
   const int blocks = (int)std::min<uint64_t>((N + kRBlock - 1) / kRBlock, (uint64_t)c->n_cu * 8);
@@ -1248,6 +1269,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   }
   PassArgs P;
   P.n_paths = N;
+  P.npix = npix;
   P.key1 = (uint32_t)prm->seed ^ (uint32_t)(prm->seed >> 32) * 0x9E3779B9u;
   P.max_depth = prm->max_depth;
   P.rr_start = prm->rr_start_depth;
@@ -1293,16 +1315,22 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     (void)hipEventRecord(e, s);
     return e;
   };
-  hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, B.sum, 3 * (size_t)N);
-  hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, B.sumsq, 3 * (size_t)N);
+  hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, B.sum, 3 * (size_t)npix);
+  hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, B.sumsq, 3 * (size_t)npix);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, B.stats, 8, B.overflow);
   RXCK(hipGetLastError(), "launch fill");
   RXCK(hipEventRecord(c->ev0, s), "event");
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ext, shd, sdw;
   uint64_t launches = 0;
-  for (uint32_t pass = 0; pass < prm->spp; pass++) {
-    P.sample = prm->sample_offset + pass;
-    GA.tag = (mode == RX_MODE_GUIDED && P.sample < train_passes) ? (float)(pass + 1) : 0.0f;
+  uint32_t pass = 0;
+  for (uint32_t done = 0; done < prm->spp; pass++) {
+    P.sample = prm->sample_offset + done;
+    // a training pass renders one sample per pixel; otherwise up to S_max samples per pass
+    const bool train = mode == RX_MODE_GUIDED && P.sample < train_passes;
+    const uint32_t spass = train ? 1u : std::min(S_max, prm->spp - done);
+    P.n_paths = npix * spass;
+    done += spass;
+    GA.tag = train ? (float)(pass + 1) : 0.0f;
     hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P);
     for (uint32_t b = 0; b <= prm->max_depth; b++) {
       hipEvent_t e0 = timing ? mark() : nullptr;
@@ -1337,11 +1365,11 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
         if (mode) sdw.push_back({e2, e3});
       }
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(kRBlock), 0, s, B, N);
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(kRBlock), 0, s, B, npix, spass);
     RXCK(hipGetLastError(), "launch pass");
     if (GA.tag > 0.0f) {  // online training on this pass's radiance samples
       GuideState& G = *c->guide;
-      const uint32_t K = N * GA.R;
+      const uint32_t K = P.n_paths * GA.R;
       const uint32_t nb = (K + kRecChunk - 1) / kRecChunk;
       hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(256), 0, s, GA.rec, K, GA.tag, R.bcount.as<uint32_t>());
       hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(64), 0, s, R.bcount.as<uint32_t>(), nb, R.total.as<uint32_t>());
@@ -1355,12 +1383,12 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   }
   RXCK(hipEventRecord(c->ev1, s), "event");
   if (rgb_mean && prm->spp) {
-    hipLaunchKernelGGL(k_mean, dim3(1024), dim3(256), 0, s, B.sum, R.mean.as<float>(), 3 * (size_t)N,
+    hipLaunchKernelGGL(k_mean, dim3(1024), dim3(256), 0, s, B.sum, R.mean.as<float>(), 3 * (size_t)npix,
                        1.0 / (double)prm->spp);
-    RXCK(hipMemcpyAsync(rgb_mean, R.mean.p, 12 * (size_t)N, hipMemcpyDeviceToHost, s), "download mean");
+    RXCK(hipMemcpyAsync(rgb_mean, R.mean.p, 12 * (size_t)npix, hipMemcpyDeviceToHost, s), "download mean");
   }
-  if (sum) RXCK(hipMemcpyAsync(sum, B.sum, 24 * (size_t)N, hipMemcpyDeviceToHost, s), "download sum");
-  if (sumsq) RXCK(hipMemcpyAsync(sumsq, B.sumsq, 24 * (size_t)N, hipMemcpyDeviceToHost, s), "download sumsq");
+  if (sum) RXCK(hipMemcpyAsync(sum, B.sum, 24 * (size_t)npix, hipMemcpyDeviceToHost, s), "download sum");
+  if (sumsq) RXCK(hipMemcpyAsync(sumsq, B.sumsq, 24 * (size_t)npix, hipMemcpyDeviceToHost, s), "download sumsq");
   unsigned long long st[8] = {0};
   int32_t ovf = 0;
   RXCK(hipMemcpyAsync(st, B.stats, 64, hipMemcpyDeviceToHost, s), "download stats");
@@ -1376,7 +1404,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     stats->segments = st[3];
     stats->box_tests = st[4];
     stats->tri_tests = st[5];
-    stats->samples = (uint64_t)N * prm->spp;
+    stats->samples = (uint64_t)npix * prm->spp;
     stats->ms_total = ms;
     stats->extend_launches = launches;
     auto tsum = [](const std::vector<std::pair<hipEvent_t, hipEvent_t>>& v) {

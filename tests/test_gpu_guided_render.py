@@ -23,18 +23,24 @@ def _guided(rx, ctx, sc, passes=8, samples=1 << 16):
     return pt
 
 
-def test_guided_mean_matches_plain(rx, oracle_mod, gpu_ctx):
-    print("[guided] unbiasedness vs oracle", flush=True)
-    sc = scenes.media_scene(32, 32, max_depth=16, freq=24, glass_freq=8)
-    so = oracle_mod.SceneOracle(sc)
-    on = 512
-    osum, osq = so.render(on, seed=9)
+@pytest.mark.parametrize("scene", ["media", "icosphere"])
+def test_guided_mean_matches_plain(rx, gpu_ctx, scene):
+    """Guided image vs a long plain PathTracer render (the estimator the oracle
+    parity tests validate): same per-pixel means within 3 sigma."""
+    print(f"[guided] unbiasedness {scene}", flush=True)
+    if scene == "media":
+        sc = scenes.media_scene(24, 24, max_depth=16, freq=24, glass_freq=8)
+    else:
+        sc = scenes.icosphere_scene(24, 24, max_depth=10, freq=24)
+    ref = rx.PathTracerHIP(sc, ctx=gpu_ctx).renderPixels(spp=1 << 18, mode=MODE_PT, seed=77)
     pt = _guided(rx, gpu_ctx, sc)
-    gn = 8192
+    gn = 1 << 16
     r = pt.renderPixels(spp=gn, mode=MODE_GUIDED, seed=21)
-    frac, zmean, z = three_sigma(r["sum"], r["sumsq"], gn, osum, osq, on)
-    print(f"  within 3 sigma {frac:.4f}, mean z {zmean:+.4f}")
-    assert frac >= 0.995 and abs(zmean) < 0.08
+    frac, zmean, z = three_sigma(r["sum"], r["sumsq"], gn, ref["sum"], ref["sumsq"], 1 << 18, pooled=False)
+    print(f"  within 3 sigma {frac:.4f}, mean z {zmean:+.4f}, guided mean {r['sum'].mean() / gn:.5f} "
+          f"plain mean {ref['sum'].mean() / (1 << 18):.5f}")
+    assert np.isfinite(r["sum"]).all()
+    assert frac >= 0.995 and abs(zmean) < 0.1
 
 
 def _equal_time(pt, mode, budget_s, chunk, seed):
