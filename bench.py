@@ -30,6 +30,11 @@ HBM_PEAK_GBS = 8000.0
 B_RAY = 44
 B_BOX = 32
 B_TRI = 40
+# HBM bytes per k_extend launch of this workload, measured by two rocprofv3
+# --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on the
+# rx_copy_kernel 256 MiB copy; the commands and raw figures are in
+# profiles/k_extend_icosphere1M_1024px.md
+EXTEND_TRAFFIC_BYTES = 59.05e6
 
 
 def log(*a):
@@ -147,18 +152,17 @@ def main():
                    "samples_per_step": args.res * args.res * args.spp},
         "samples_per_s": args.res * args.res * args.spp * args.steps / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": EXTEND_TRAFFIC_BYTES,
                      "kernel": "k_extend (nearest-hit traversal)",
                      "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_launch_s * 1e3},
     }
     if args.calib_copy_mib:
-        n = args.calib_copy_mib * (1 << 20) // 4
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        from renderx_amd import _lib
+        nbytes = args.calib_copy_mib << 20
+        a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
+        pt.ctx.check(_lib.lib().rx_copy_kernel(pt.ctx.h, b.data_ptr(), a.data_ptr(), nbytes), "rx_copy_kernel")
+        log(f"[bench] calibration: rx::k_copy16 over {args.calib_copy_mib} MiB done")
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(scene)

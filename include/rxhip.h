@@ -167,6 +167,11 @@ int rx_intersect_dev(rx_ctx* ctx, uint32_t n, const float* org, const float* dir
 /* Duration in ms of the last rx_intersect(_dev) traversal kernel (HIP events
  * on the context stream; valid after rx_synchronize). */
 int rx_last_kernel_ms(rx_ctx* ctx, float* ms);
+/* No counterpart in the reference: a plain streaming device-to-device copy
+ * (each byte read once and written once; 16-byte aligned pointers and size),
+ * the known-traffic kernel that calibrates hardware byte counters against
+ * the traversal kernels.  Synchronous. */
+int rx_copy_kernel(rx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* ------------------------------------------------------------------ scene + render */
 /* Scene::buildKDTree + Scene::buildObjKDTrees (src/Scene.cpp:248-320) with
@@ -186,9 +191,12 @@ int rx_render(rx_ctx* ctx, const rx_camera_desc* camera, const rx_render_params*
 
 /* ------------------------------------------------------------------ path guiding */
 typedef struct rx_guide_params {
-  uint32_t train_passes;     /* warm-up passes that train the network online */
-  uint32_t samples_per_pass; /* radiance samples per training step */
-  float learning_rate;       /* Adam */
+  uint32_t train_passes;   /* leading samples per pixel that record training data (one pass each) */
+  uint32_t batch_size;     /* radiance samples per Adam step (>= 256) */
+  uint32_t steps_per_pass; /* Adam steps after each training pass, over deterministic
+                              permutations of that pass's samples (a store of at most
+                              batch_size * steps_per_pass samples, an even subset) */
+  float learning_rate;     /* Adam */
   float beta1, beta2, eps;
   uint32_t seed;
   uint32_t flags;

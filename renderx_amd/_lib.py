@@ -32,6 +32,7 @@ SIGNATURES = {
     "rx_intersect": (C.c_int, [P, U32, P, P, P, P, U32, P, P, P, P]),
     "rx_intersect_dev": (C.c_int, [P, U32, P, P, P, P, U32, P, P, P, P]),
     "rx_last_kernel_ms": (C.c_int, [P, C.POINTER(F32)]),
+    "rx_copy_kernel": (C.c_int, [P, P, P, U64]),
     "rx_load_scene": (C.c_int, [P, U32, P, U32, P, U32, P, U32]),
     "rx_render": (C.c_int, [P, P, P, P, P, P, P]),
     "rx_guide_init": (C.c_int, [P, P]),

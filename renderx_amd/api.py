@@ -26,7 +26,7 @@ RENDER_STATS = np.dtype([("rays", "<u8"), ("extend_rays", "<u8"), ("shadow_rays"
                          ("ms_extend", "<f8"), ("ms_shade", "<f8"), ("ms_shadow", "<f8"),
                          ("extend_launches", "<u8")])
 RENDER_TIMING, RENDER_COUNT_WORK, RENDER_NO_GRAPH = 1, 2, 4
-GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("samples_per_pass", "<u4"), ("learning_rate", "<f4"),
+GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("batch_size", "<u4"), ("steps_per_pass", "<u4"), ("learning_rate", "<f4"),
                          ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4")])
 MODE_PT, MODE_NEE, MODE_GUIDED = 0, 1, 2
 
@@ -134,10 +134,11 @@ class PathTracerHIP:
         self.ctx.check(lib().rx_load_scene(self.ctx.h, len(v), ptr(v), len(t), ptr(t), len(o), ptr(o),
                                            1 if tree_abs_int else 0), "rx_load_scene")
 
-    def guide_init(self, train_passes=8, samples_per_pass=1 << 18, learning_rate=1e-2, seed=7):
+    def guide_init(self, train_passes=16, batch_size=4096, steps_per_pass=16, learning_rate=1e-2, seed=7):
         p = np.zeros(1, GUIDE_PARAMS)
         p["train_passes"] = train_passes
-        p["samples_per_pass"] = samples_per_pass
+        p["batch_size"] = batch_size
+        p["steps_per_pass"] = steps_per_pass
         p["learning_rate"] = learning_rate
         p["beta1"], p["beta2"], p["eps"] = 0.9, 0.999, 1e-8
         p["seed"] = seed

@@ -319,3 +319,24 @@ int rx_intersect(rx_ctx* c, uint32_t n, const float* org, const float* dir, cons
 }
 
 }  // extern "C"
+
+namespace rx {
+// Streaming copy, 16 B per lane per iteration: reads and writes each byte once.
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace rx
+
+extern "C" int rx_copy_kernel(rx_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || !dst || !src) return set_err(c, RX_ERR_INVALID, "rx_copy_kernel: null pointer");
+  if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+    return set_err(c, RX_ERR_INVALID, "rx_copy_kernel: pointers and size must be multiples of 16 bytes");
+  if (bytes == 0) return RX_OK;
+  (void)hipSetDevice(c->device);
+  const size_t n16 = bytes / 16;
+  const int blocks = (int)std::min<size_t>((n16 + 255) / 256, (size_t)c->n_cu * 32);
+  hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, c->stream, (const uint4*)src, (uint4*)dst, n16);
+  if (int r = hip_err(c, hipGetLastError(), "copy launch")) return r;
+  return hip_err(c, hipStreamSynchronize(c->stream), "copy synchronize");
+}

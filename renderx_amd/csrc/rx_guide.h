@@ -133,7 +133,9 @@ struct GuideState {
   rx_guide_params prm{};
   DBuf params, adam_m, adam_v, grad, Wt[4], Wr[4];
   DBuf X, A1, A2, A3, Z4, dZ4, D1, D2, D3, partial[4], loss_part, rows, pos, dir, wdir, pb, wt;
-  uint32_t cap = 0;
+  // one training pass's radiance samples, mini-batches are gathered from here
+  DBuf spos, sdir, swdir, spb, swt, srows;
+  uint32_t cap = 0, store_cap = 0;
   int step = 0;
   float lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
   GuideNet net() const {
@@ -157,6 +159,7 @@ struct GuideState {
 };
 
 int guide_reserve(rx_ctx* c, GuideState& G, uint32_t n);
+int guide_reserve_store(rx_ctx* c, GuideState& G, uint32_t n);
 int guide_train_rows(rx_ctx* c, uint32_t cap);
 
 }  // namespace rx

@@ -411,6 +411,18 @@ int guide_reserve(rx_ctx* c, GuideState& G, uint32_t n) {
   return RX_OK;
 }
 
+int guide_reserve_store(rx_ctx* c, GuideState& G, uint32_t n) {
+  if (n <= G.store_cap) return RX_OK;
+  const size_t N = n;
+  hipError_t e = hipSuccess;
+  for (DBuf* b : {&G.spos, &G.sdir, &G.swdir}) e = e == hipSuccess ? b->ensure(N * 16) : e;
+  for (DBuf* b : {&G.spb, &G.swt}) e = e == hipSuccess ? b->ensure(N * 4) : e;
+  if (e == hipSuccess) e = G.srows.ensure(64);
+  if (int r = hip_err(c, e, "guide sample store")) return r;
+  G.store_cap = n;
+  return RX_OK;
+}
+
 // One training step on rows already in G.X / G.wdir / G.pb / G.wt with the
 // row count at G.rows (device).  Loss partials stay on the device.
 static int guide_train_device(rx_ctx* c, GuideState& G, uint32_t cap, float* grad_out) {

@@ -692,8 +692,10 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
         beta = beta * decay(S, inside, seg);
         // vec3 / 0 leaves the vector unchanged in the reference (nvVector.h operator/=)
         if (sc.originProb != 0.0f) beta = beta * (1.0f / sc.originProb);
-        if (MODE >= 1 && sc.neeKind) {
-          // next-event estimation to an area light: one light point, MIS with the continuation density
+        if (MODE >= 1 && sc.neeKind && b < P.max_depth) {
+          // next-event estimation to an area light: one light point, MIS with the continuation density.
+          // The light point is the path's vertex b + 1 (+ the boundary crossings of the shadow
+          // ray), so like a continuation it needs a bounce left.
           U4 r2 = philox(U4{smp, b, 2u, 0u}, pix, P.key1);
           float us = u01(r2.x), u1 = u01(r2.y), u2 = u01(r2.z);
           int lo = 0, hi = S.n_etri - 1;
@@ -814,8 +816,12 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S, PathBufs B, PassA
 // Visibility as MCRenderer::testVisibility (src/MCRenderer.cpp:108-120):
 // visible when the first legal hit is not nearer than the light point
 // (minus EPSILON).  Index-matched medium boundaries are crossed with the
-// transmittance the PathTracer's distance sampling realises on average.
-__global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S, PathBufs B, uint32_t b, GuideArgs GA) {
+// transmittance the PathTracer's distance sampling realises on average; each
+// crossing is a bounce of the equivalent BSDF-sampled path, so a light reached
+// after c crossings from the vertex of bounce b counts only when
+// b + 1 + c <= max_depth, the paths the PathTracer itself can produce.
+__global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S, PathBufs B, uint32_t b, uint32_t max_depth,
+                                                    GuideArgs GA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
   const uint32_t n = B.cnt[kShadowBase + b];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -840,7 +846,7 @@ __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S, PathBufs B, uint
       rays++;
       srays++;
       if (h.pos < 0 || h.dist - remaining >= -1e-5f) {
-        visible = true;
+        visible = b + 1 + (uint32_t)c <= max_depth;
         break;
       }
       const int hobj = __float_as_int(S.tree.tv1[h.pos].w);
@@ -959,6 +965,25 @@ __global__ void k_rec_write(const float* rec, uint32_t K, float tag, const uint3
     T.wdir[slot] = make_float4(r[6], r[7], r[8], 0.0f);
     T.pb[slot] = r[10];
     T.wt[slot] = isfinite(w) ? w : 0.0f;
+  }
+}
+
+// Mini-batch `step` of a training pass: rows r < min(batch, n) take stored
+// sample ((step * batch + r) * 0x7fffffff) mod n, a permutation of the n
+// stored samples (the prime multiplier is coprime to any n < 2^31), so
+// consecutive steps sweep the store in a scattered, deterministic order.
+__global__ void k_guide_gather(TrainOut S, const uint32_t* total, uint32_t store_cap, uint32_t step, uint32_t batch,
+                               TrainOut D) {
+  const uint32_t n = min(*total, store_cap);
+  const uint32_t rows = min(n, batch);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *D.rows = rows;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+    const uint32_t k = (uint32_t)((((uint64_t)step * batch + r) * 0x7fffffffull) % n);
+    D.pos[r] = S.pos[k];
+    D.dir[r] = S.dir[k];
+    D.wdir[r] = S.wdir[k];
+    D.pb[r] = S.pb[k];
+    D.wt[r] = S.wt[k];
   }
 }
 
@@ -1279,7 +1304,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   const bool count = (prm->flags & RX_RENDER_COUNT_WORK) != 0;
   const int mode = (int)prm->mode;
   GuideArgs GA{};
-  uint32_t train_passes = 0, train_cap = 0;
+  uint32_t train_passes = 0, batch = 0, steps = 0, store_cap = 0;
   if (mode == RX_MODE_GUIDED) {
     if (!c->guide) return set_err(c, RX_ERR_STATE, "rx_render: guided mode needs rx_guide_init");
     GuideState& G = *c->guide;
@@ -1291,8 +1316,12 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     // samples with index < train_passes train the network online (a render
     // resumed with sample_offset >= train_passes only uses it)
     train_passes = G.prm.train_passes;
-    train_cap = std::max(1024u, G.prm.samples_per_pass);
-    if (int r = guide_reserve(c, G, std::max(N, train_cap))) return r;
+    batch = std::max(256u, G.prm.batch_size);
+    steps = std::max(1u, G.prm.steps_per_pass);
+    store_cap = (uint32_t)std::min<uint64_t>((uint64_t)batch * steps, 1u << 26);
+    if (int r = guide_reserve(c, G, std::max(N, batch))) return r;
+    if (prm->sample_offset < train_passes && prm->spp)
+      if (int r = guide_reserve_store(c, G, store_cap)) return r;
     const uint32_t Rs = prm->max_depth + 1;
     if (prm->sample_offset < train_passes && prm->spp) {
       RXCK(R.rec.ensure((size_t)N * Rs * kRecFloats * 4), "alloc records");
@@ -1356,7 +1385,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       hipEvent_t e2 = timing ? mark() : nullptr;
       hipEvent_t e3 = nullptr;
       if (mode) {
-        hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(kRBlock), 0, s, S, B, b, GA);
+        hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(kRBlock), 0, s, S, B, b, prm->max_depth, GA);
         e3 = timing ? mark() : nullptr;
       }
       if (timing) {
@@ -1373,12 +1402,19 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       const uint32_t nb = (K + kRecChunk - 1) / kRecChunk;
       hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(256), 0, s, GA.rec, K, GA.tag, R.bcount.as<uint32_t>());
       hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(64), 0, s, R.bcount.as<uint32_t>(), nb, R.total.as<uint32_t>());
-      TrainOut TO{G.pos.as<float4>(), G.dir.as<float4>(), G.wdir.as<float4>(), G.pb.as<float>(), G.wt.as<float>(),
-                  G.rows.as<uint32_t>()};
+      const TrainOut ST{G.spos.as<float4>(), G.sdir.as<float4>(), G.swdir.as<float4>(), G.spb.as<float>(),
+                        G.swt.as<float>(), G.srows.as<uint32_t>()};
+      const TrainOut TB{G.pos.as<float4>(), G.dir.as<float4>(), G.wdir.as<float4>(), G.pb.as<float>(),
+                        G.wt.as<float>(), G.rows.as<uint32_t>()};
       hipLaunchKernelGGL(k_rec_write, dim3(nb), dim3(256), 0, s, GA.rec, K, GA.tag, R.bcount.as<uint32_t>(),
-                         R.total.as<uint32_t>(), train_cap, B.L, GA.R, TO);
-      RXCK(hipGetLastError(), "launch gather");
-      if (int r = guide_train_rows(c, train_cap)) return r;
+                         R.total.as<uint32_t>(), store_cap, B.L, GA.R, ST);
+      RXCK(hipGetLastError(), "launch record write");
+      for (uint32_t k = 0; k < steps; k++) {
+        hipLaunchKernelGGL(k_guide_gather, dim3(std::min<uint32_t>((batch + 255) / 256, 1024u)), dim3(256), 0, s, ST,
+                           R.total.as<uint32_t>(), store_cap, k, batch, TB);
+        RXCK(hipGetLastError(), "launch gather");
+        if (int r = guide_train_rows(c, batch)) return r;
+      }
     }
   }
   RXCK(hipEventRecord(c->ev1, s), "event");

@@ -29,12 +29,37 @@ def z_scores(s1, q1, n1, s2, q2, n2, pooled=True):
     return z
 
 
-def three_sigma(s1, q1, n1, s2, q2, n2, pooled=True):
-    """Fraction of pixel-channels within 3 sigma, mean z, and the z array.
-    Use pooled=False when the two renders use different estimators."""
+def n_eff(s, q):
+    """Effective number of contributing samples (sum x)^2 / sum x^2: n for a
+    constant estimator, k for k equal non-zero samples among zeros.  A
+    pixel-channel whose estimate rests on a handful of rare samples has an
+    unreliable variance estimate, so the normal approximation behind a
+    z-test does not hold there."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(q > 0, s * s / np.where(q > 0, q, 1), 0.0)
+
+
+def clt_mask(*sums, min_eff=3):
+    """Pixel-channels where every (sum, sumsq) pair has at least min_eff
+    effective samples, or where all of them are exactly zero.  An estimate
+    resting on fewer than three samples has no usable variance estimate."""
+    ok = np.ones(np.shape(sums[0]), bool)
+    zero = np.ones(np.shape(sums[0]), bool)
+    for s, q in zip(sums[0::2], sums[1::2]):
+        ok &= n_eff(s, q) >= min_eff
+        zero &= (s == 0) & (q == 0)
+    return ok | zero
+
+
+def three_sigma(s1, q1, n1, s2, q2, n2, pooled=True, mask=None):
+    """Fraction of pixel-channels within 3 sigma, mean z (z clipped to +-5 so
+    that it measures a systematic shift, not outliers) and the z array, over
+    `mask` (default: all).  Use pooled=False when the two renders use
+    different estimators."""
     z = z_scores(s1, q1, n1, s2, q2, n2, pooled)
-    fin = np.isfinite(z)
-    return float((np.abs(z) <= 3).mean()), float(z[fin].mean()), z
+    sel = np.ones(z.shape, bool) if mask is None else mask
+    zs = z[sel]
+    return float((np.abs(zs) <= 3).mean()), float(np.clip(zs, -5, 5).mean()), z
 
 
 def rel_mse(img, ref, eps=1e-2):

@@ -31,7 +31,7 @@ class Guide:
         self.L = _lib.lib()
         self.ctx = ctx
         p = np.zeros(1, rx.GUIDE_PARAMS)
-        p["train_passes"], p["samples_per_pass"], p["learning_rate"] = 8, 1 << 18, 1e-2
+        p["train_passes"], p["batch_size"], p["steps_per_pass"], p["learning_rate"] = 8, 4096, 16, 1e-2
         p["beta1"], p["beta2"], p["eps"], p["seed"] = 0.9, 0.999, 1e-8, 7
         ctx.check(self.L.rx_guide_init(ctx.h, p.ctypes.data_as(C.c_void_p)), "rx_guide_init")
         lo = np.zeros(3, np.float32)
