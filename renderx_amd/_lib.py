@@ -10,6 +10,9 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librxhip.so")
+# RENDERX_AMD_LIB selects another build of the same library (e.g. one compiled
+# with a different -DRX_LDS_DEPTH) without touching the in-tree one.
+LIB_PATH = os.environ.get("RENDERX_AMD_LIB", LIB_PATH)
 
 RX_OK = 0
 STATUS = {0: "RX_OK", -1: "RX_ERR_INVALID", -2: "RX_ERR_HIP", -3: "RX_ERR_NO_MEMORY", -4: "RX_ERR_STATE",

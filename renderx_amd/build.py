@@ -34,22 +34,26 @@ def _run(cmd):
     return r
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = False, out: str = OUT, defines: tuple = ()) -> str:
+    """Compile every source and link `out`; `defines` (e.g. "RX_LDS_DEPTH=12")
+    go to the HIP compiles, with their own object folder."""
+    build_dir = BUILD if not defines else BUILD + "_" + "_".join(d.replace("=", "") for d in defines)
+    os.makedirs(build_dir, exist_ok=True)
+    dflags = ["-D" + d for d in defines]
     jobs = []
     for s in HIP_SOURCES:
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", os.path.join(BUILD, s + ".o")])
+        jobs.append([HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, s), "-o", os.path.join(build_dir, s + ".o")])
     for s in CXX_SOURCES:
-        jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o", os.path.join(BUILD, s + ".o")])
+        jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o", os.path.join(build_dir, s + ".o")])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)
-    objs = [os.path.join(BUILD, s + ".o") for s in HIP_SOURCES + CXX_SOURCES]
-    tmp = OUT + ".tmp"
+    objs = [os.path.join(build_dir, s + ".o") for s in HIP_SOURCES + CXX_SOURCES]
+    tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

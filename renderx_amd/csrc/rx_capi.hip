@@ -91,9 +91,12 @@ int upload_tree(rx_ctx* c) {
   return RX_OK;
 }
 
+// All blocks of a traversal launch are resident at once: each block runs a
+// fixed range of the batch, so a block that started late would finish late.
 static int max_blocks(rx_ctx* c, int block) {
   (void)block;
-  return c->n_cu * 16;
+  if (!c->isect_blocks_per_cu) c->isect_blocks_per_cu = intersect_blocks_per_cu();
+  return c->n_cu * c->isect_blocks_per_cu;
 }
 
 }  // namespace rx

@@ -36,26 +36,26 @@ struct Hit {
 struct Vec3 {
   float x, y, z;
 };
-__device__ __forceinline__ Vec3 v3(float a, float b, float c) { return Vec3{a, b, c}; }
-__device__ __forceinline__ Vec3 operator+(Vec3 a, Vec3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ Vec3 operator-(Vec3 a, Vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ Vec3 operator-(Vec3 a) { return v3(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ Vec3 operator*(Vec3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ Vec3 operator*(float s, Vec3 a) { return v3(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ Vec3 operator*(Vec3 a, Vec3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ Vec3 cross(Vec3 l, Vec3 r) {
+__host__ __device__ __forceinline__ Vec3 v3(float a, float b, float c) { return Vec3{a, b, c}; }
+__host__ __device__ __forceinline__ Vec3 operator+(Vec3 a, Vec3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ __forceinline__ Vec3 operator-(Vec3 a, Vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ __forceinline__ Vec3 operator-(Vec3 a) { return v3(-a.x, -a.y, -a.z); }
+__host__ __device__ __forceinline__ Vec3 operator*(Vec3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ __forceinline__ Vec3 operator*(float s, Vec3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__host__ __device__ __forceinline__ Vec3 operator*(Vec3 a, Vec3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__host__ __device__ __forceinline__ Vec3 cross(Vec3 l, Vec3 r) {
   return v3(l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x);
 }
 // ((0 + a.x*b.x) + a.y*b.y) + a.z*b.z: the leading 0 + turns -0 into +0 like the reference
-__device__ __forceinline__ float dot(Vec3 a, Vec3 b) {
+__host__ __device__ __forceinline__ float dot(Vec3 a, Vec3 b) {
   float r = 0.0f;
   r += a.x * b.x;
   r += a.y * b.y;
   r += a.z * b.z;
   return r;
 }
-__device__ __forceinline__ float length(Vec3 a) { return __builtin_sqrtf(dot(a, a)); }
-__device__ __forceinline__ Vec3 normalize(Vec3 a) {
+__host__ __device__ __forceinline__ float length(Vec3 a) { return __builtin_sqrtf(dot(a, a)); }
+__host__ __device__ __forceinline__ Vec3 normalize(Vec3 a) {
   float r = length(a);
   if (r > 0) return v3(a.x / r, a.y / r, a.z / r);
   return a;
@@ -67,7 +67,7 @@ __device__ __forceinline__ float comp(Vec3 a, int i) { return i == 0 ? a.x : (i 
 // KDTree::Node::BoundingBox::intersect (src/KDTree.cpp:34-52) reduced to the
 // only thing the reference reads from it: minDist >= 0, i.e. some face passes.
 // Faces are independent, so the loop may stop at the first passing face.
-__device__ __forceinline__ bool face_pass(float plane, float o_a, float d_a, float o_b, float d_b, float lo_b,
+__host__ __device__ __forceinline__ bool face_pass(float plane, float o_a, float d_a, float o_b, float d_b, float lo_b,
                                           float hi_b, float o_c, float d_c, float lo_c, float hi_c) {
   if (d_a == 0) return false;
   float dist = (plane - o_a) / d_a;
@@ -75,7 +75,7 @@ __device__ __forceinline__ bool face_pass(float plane, float o_a, float d_a, flo
   float pc = o_c + dist * d_c;
   return dist >= 0 && lo_b <= pb && hi_b >= pb && lo_c <= pc && hi_c >= pc;
 }
-__device__ __forceinline__ bool box_hit(const float lo[3], const float hi[3], Vec3 o, Vec3 d) {
+__host__ __device__ __forceinline__ bool box_hit(const float lo[3], const float hi[3], Vec3 o, Vec3 d) {
   // faces i = 0..5: plane ((float*)this)[i], axis a = i%3, checks (i+1)%3, (i+2)%3
   if (face_pass(lo[0], o.x, d.x, o.y, d.y, lo[1], hi[1], o.z, d.z, lo[2], hi[2])) return true;
   if (face_pass(lo[1], o.y, d.y, o.z, d.z, lo[2], hi[2], o.x, d.x, lo[0], hi[0])) return true;
@@ -87,7 +87,7 @@ __device__ __forceinline__ bool box_hit(const float lo[3], const float hi[3], Ve
 
 // Slab entry distance with per-ray reciprocals: ordering key only (not a hit
 // test).  Sub then mul, compare-select; identical to the oracle's key.
-__device__ __forceinline__ float slab_tnear(const float lo[3], const float hi[3], Vec3 o, Vec3 d, Vec3 inv) {
+__host__ __device__ __forceinline__ float slab_tnear(const float lo[3], const float hi[3], Vec3 o, Vec3 d, Vec3 inv) {
   const float INF = __builtin_inff();
   float t0, t1, a, b, c;
   if (d.x == 0) { t0 = -INF; t1 = INF; } else { t0 = (lo[0] - o.x) * inv.x; t1 = (hi[0] - o.x) * inv.x; }
@@ -100,6 +100,49 @@ __device__ __forceinline__ float slab_tnear(const float lo[3], const float hi[3]
   t = b > t ? b : t;
   t = c > t ? c : t;
   return t;
+}
+
+// Certain-miss filter for box_hit.  If box_hit passes a face of axis c at
+// dist t, the real hit point t* = (plane - o_c) / d_c >= 0 lies in every
+// slab a widened by the rounding of the face check, |p_a - p_a*| <=
+// 4u (|t*| |d_a| + B_a) (u = 2^-24, B_a = max |coordinate| of the scene box):
+// in t units e_a - 4u t* - beta_a <= t* <= x_a + 4u t* + beta_a with
+// beta_a = 4u B_a / |d_a| (+ an absolute floor).  The slab ends e_a, x_a are
+// computed here with reciprocals (relative error <= 3u, bounded by 8u with
+// this function's own rounding), so when no t >= 0 satisfies the widened
+// bounds, box_hit is certainly false and may be skipped.  Needs
+// |d_a| >= 2^-30 on every axis (no overflow, no infinities); `beta` holds
+// 16u B_a / |d_a| + 2^-120 / |d_a|.  The key is slab_tnear's value.
+__host__ __device__ __forceinline__ bool slab_certain_miss(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
+                                                           Vec3 beta, float& tnear) {
+  const float U8 = 8.0f * 0x1p-24f;
+  const float t0x = (lo[0] - o.x) * inv.x, t1x = (hi[0] - o.x) * inv.x;
+  const float t0y = (lo[1] - o.y) * inv.y, t1y = (hi[1] - o.y) * inv.y;
+  const float t0z = (lo[2] - o.z) * inv.z, t1z = (hi[2] - o.z) * inv.z;
+  const float ex = t0x < t1x ? t0x : t1x, xx = t0x < t1x ? t1x : t0x;
+  const float ey = t0y < t1y ? t0y : t1y, xy = t0y < t1y ? t1y : t0y;
+  const float ez = t0z < t1z ? t0z : t1z, xz = t0z < t1z ? t1z : t0z;
+  float t = ex;
+  t = ey > t ? ey : t;
+  t = ez > t ? ez : t;
+  tnear = t;
+  const float lx = fmaf(-U8, fabsf(ex), ex) - beta.x, hxx = fmaf(U8, fabsf(xx), xx) + beta.x;
+  const float ly = fmaf(-U8, fabsf(ey), ey) - beta.y, hxy = fmaf(U8, fabsf(xy), xy) + beta.y;
+  const float lz = fmaf(-U8, fabsf(ez), ez) - beta.z, hxz = fmaf(U8, fabsf(xz), xz) + beta.z;
+  const float L = fmaxf(fmaxf(0.0f, lx), fmaxf(ly, lz));
+  const float R = fminf(hxx, fminf(hxy, hxz));
+  return L * (1.0f - 16.0f * 0x1p-24f) > R * (1.0f + 16.0f * 0x1p-24f);
+}
+// Per-ray set-up of slab_certain_miss; false when the filter must not be used.
+__host__ __device__ __forceinline__ bool slab_filter_setup(const float root_lo[3], const float root_hi[3], Vec3 d,
+                                                           Vec3 inv, Vec3& beta) {
+  const float m = 0x1p-30f;
+  if (!(fabsf(d.x) >= m && fabsf(d.y) >= m && fabsf(d.z) >= m)) return false;
+  const float c = 16.0f * 0x1p-24f;
+  beta = v3((c * fmaxf(fabsf(root_lo[0]), fabsf(root_hi[0])) + 0x1p-120f) * fabsf(inv.x) * 1.0001f,
+            (c * fmaxf(fabsf(root_lo[1]), fabsf(root_hi[1])) + 0x1p-120f) * fabsf(inv.y) * 1.0001f,
+            (c * fmaxf(fabsf(root_lo[2]), fabsf(root_hi[2])) + 0x1p-120f) * fabsf(inv.z) * 1.0001f);
+  return true;
 }
 
 // KDTree::intersect(ray, triangle) (src/KDTree.cpp:267-293) without the
@@ -144,100 +187,191 @@ struct TraceStats {
 //
 // stack: per-lane LDS column of LDS_DEPTH entries (stride = blockDim.x),
 // spill: per-thread global overflow of SPILL_DEPTH entries.
+// Traversal state of one ray, so that a lane can run it in bounded trips
+// (descend to a leaf, test the leaf, pop) and take a new ray when it ends.
+struct TraceState {
+  Vec3 o, d, inv, beta;
+  bool fast, has;
+  float best;
+  int32_t bestLeaf, bestPos, ref, inside, contact;
+  int sp;
+};
+
+// Root box test and set-up; false when the ray misses the scene box.
+template <bool STATS>
+__device__ __forceinline__ bool trace_begin(const DevTree& T, TraceState& s, Vec3 o, Vec3 d, int32_t inside,
+                                            int32_t contact, TraceStats* st) {
+  if (STATS) { st->boxes = 1; st->tris = 0; }
+  s.o = o;
+  s.d = d;
+  s.inside = inside;
+  s.contact = contact;
+  s.has = false;
+  s.best = -1.0f;
+  s.bestLeaf = s.bestPos = -1;
+  s.sp = 0;
+  s.ref = T.root_ref;
+  if (!box_hit(T.root_lo, T.root_hi, o, d)) return false;
+  s.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  s.beta = v3(0, 0, 0);
+  s.fast = slab_filter_setup(T.root_lo, T.root_hi, d, s.inv, s.beta);
+  return true;
+}
+
+// One trip: descend interior nodes to a leaf (or a dead end), test the
+// leaf's triangles, pop the next subtree.  Returns true when the traversal
+// has ended (s.has / s.best / s.bestPos hold the result).  EXHAUSTIVE visits
+// every child whose box passes (the reference recursion's visit set);
+// otherwise the nearer child (slab key) goes first and subtrees whose key
+// exceeds best*1.001 are culled.  Both give the reference's winner: among
+// equal distances the larger DFS leaf index wins (right subtree wins,
+// src/KDTree.cpp:317-321), inside a leaf the first triangle wins (:305).
+//
+// lds_stack: per-lane LDS column of LDS_DEPTH entries (stride = blockDim.x),
+// spill: per-thread global overflow of spill_depth entries.
+template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH>
+__device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint2* lds_stack, uint2* spill,
+                                           int spill_depth, TraceStats* st, bool* overflow) {
+  const float INF = __builtin_inff();
+  const int stride = blockDim.x;
+  int32_t ref = s.ref;
+  while (ref >= 0) {
+    const float4* np = T.nodes + 4 * (size_t)ref;
+    float4 a = np[0], b = np[1], c = np[2];
+    int4 r = *reinterpret_cast<const int4*>(np + 3);
+    const float llo[3] = {a.x, a.y, a.z}, lhi[3] = {a.w, b.x, b.y};
+    const float rlo[3] = {b.z, b.w, c.x}, rhi[3] = {c.y, c.z, c.w};
+    if (STATS) st->boxes += 2;
+    bool hl, hr;
+    float tl = 0.0f, tr = 0.0f;
+    if (s.fast) {  // the exact predicate only where the slab filter cannot rule the box out
+      hl = !slab_certain_miss(llo, lhi, s.o, s.inv, s.beta, tl) && box_hit(llo, lhi, s.o, s.d);
+      hr = !slab_certain_miss(rlo, rhi, s.o, s.inv, s.beta, tr) && box_hit(rlo, rhi, s.o, s.d);
+    } else {
+      hl = box_hit(llo, lhi, s.o, s.d);
+      hr = box_hit(rlo, rhi, s.o, s.d);
+      if (!EXHAUSTIVE) {
+        tl = slab_tnear(llo, lhi, s.o, s.d, s.inv);
+        tr = slab_tnear(rlo, rhi, s.o, s.d, s.inv);
+      }
+    }
+    int32_t next = INT32_MIN;
+    if (EXHAUSTIVE) {
+      if (hl && hr) {
+        const uint2 e = make_uint2((uint32_t)r.y, 0u);
+        if (s.sp < LDS_DEPTH) lds_stack[s.sp * stride] = e;
+        else if (s.sp - LDS_DEPTH < spill_depth) spill[s.sp - LDS_DEPTH] = e;
+        else { *overflow = true; s.has = false; return true; }
+        s.sp++;
+        next = r.x;
+      } else if (hl) next = r.x;
+      else if (hr) next = r.y;
+    } else {
+      const float thr = s.has ? s.best * 1.001f : INF;
+      const bool vl = hl && tl <= thr, vr = hr && tr <= thr;
+      if (vl && vr) {
+        const int32_t nearRef = (tr < tl) ? r.y : r.x;
+        const int32_t farRef = (tr < tl) ? r.x : r.y;
+        const float farT = (tr < tl) ? tl : tr;
+        const uint2 e = make_uint2((uint32_t)farRef, __float_as_uint(farT));
+        if (s.sp < LDS_DEPTH) lds_stack[s.sp * stride] = e;
+        else if (s.sp - LDS_DEPTH < spill_depth) spill[s.sp - LDS_DEPTH] = e;
+        else { *overflow = true; s.has = false; return true; }
+        s.sp++;
+        next = nearRef;
+      } else if (vl) next = r.x;
+      else if (vr) next = r.y;
+    }
+    ref = next;
+  }
+  if (ref != INT32_MIN) {  // a leaf
+    const int32_t leaf = ~ref;
+    const int32_t p0 = T.leaf_start[leaf], p1 = T.leaf_start[leaf + 1];
+    for (int32_t p = p0; p < p1; p++) {
+      if (STATS) st->tris++;
+      float4 a = T.tv0[p], b = T.tv1[p], c = T.tv2[p];
+      float dist = tri_dist(xyz(a), xyz(b), xyz(c), s.o, s.d);
+      if (!(dist >= 0)) continue;
+      bool win = !s.has || dist < s.best || (dist == s.best && leaf > s.bestLeaf);
+      if (!win) continue;
+      if (LEGALITY) {
+        float4 n = T.tn[p];
+        if (!legal(s.d, xyz(n), __float_as_int(b.w), s.inside, s.contact)) continue;
+      }
+      s.has = true;
+      s.best = dist;
+      s.bestLeaf = leaf;
+      s.bestPos = p;
+    }
+  }
+  // pop
+  ref = INT32_MIN;
+  while (s.sp > 0) {
+    s.sp--;
+    const uint2 e = (s.sp < LDS_DEPTH) ? lds_stack[s.sp * stride] : spill[s.sp - LDS_DEPTH];
+    if (EXHAUSTIVE || !s.has || __uint_as_float(e.y) <= s.best * 1.001f) {
+      ref = (int32_t)e.x;
+      break;
+    }
+  }
+  s.ref = ref;
+  return ref == INT32_MIN;
+}
+
+// Nearest hit of the reference tree (trace_begin + trips until done).
 template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH>
 __device__ __forceinline__ Hit trace(const DevTree& T, Vec3 o, Vec3 d, int32_t inside, int32_t contact,
                                      uint2* lds_stack, uint2* spill, int spill_depth, TraceStats* st,
                                      bool* overflow) {
-  Hit h{-1.0f, -1};
-  if (STATS) { st->boxes = 1; st->tris = 0; }
-  if (!box_hit(T.root_lo, T.root_hi, o, d)) return h;
-  const Vec3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  const float INF = __builtin_inff();
-  bool has = false;
-  float best = -1.0f;
-  int32_t bestLeaf = -1, bestPos = -1;
-  int sp = 0;
-  int32_t ref = T.root_ref;
-  const int stride = blockDim.x;
+  TraceState s;
+  if (!trace_begin<STATS>(T, s, o, d, inside, contact, st)) return Hit{-1.0f, -1};
+  while (!trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH>(T, s, lds_stack, spill, spill_depth, st, overflow)) {
+  }
+  return s.has ? Hit{s.best, s.bestPos} : Hit{-1.0f, -1};
+}
+
+// Runs the queue entries [lo, hi) of one block: each lane holds one ray and
+// advances it a trip at a time; lanes whose ray has ended take the next
+// entry through the block's LDS counter *blk_next (initialised to lo by the
+// caller before a barrier), so a long ray holds up only its own lane.
+// fetch(i, o, d, inside, contact) loads entry i; emit(i, hit, stats) stores
+// its result.  Every wave leaves once the range is drained and its lanes
+// are idle.
+template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, class Fetch, class Emit>
+__device__ __forceinline__ void trace_range(const DevTree& T, uint32_t hi, uint32_t* blk_next, uint2* lds_stack,
+                                            uint2* spill, int spill_depth, bool* overflow, Fetch fetch, Emit emit) {
+  const int lane = threadIdx.x & 63;
+  bool active = false, drained = false;
+  uint32_t cur = 0;
+  TraceState ts;
+  TraceStats st{0u, 0u};
   for (;;) {
-    // descend interior nodes
-    while (ref >= 0) {
-      const float4* np = T.nodes + 4 * (size_t)ref;
-      float4 a = np[0], b = np[1], c = np[2];
-      int4 r = *reinterpret_cast<const int4*>(np + 3);
-      const float llo[3] = {a.x, a.y, a.z}, lhi[3] = {a.w, b.x, b.y};
-      const float rlo[3] = {b.z, b.w, c.x}, rhi[3] = {c.y, c.z, c.w};
-      if (STATS) st->boxes += 2;
-      bool hl = box_hit(llo, lhi, o, d);
-      bool hr = box_hit(rlo, rhi, o, d);
-      int32_t next = INT32_MIN;
-      if (EXHAUSTIVE) {
-        if (hl && hr) {
-          if (sp < LDS_DEPTH) lds_stack[sp * stride] = make_uint2((uint32_t)r.y, 0u);
-          else if (sp - LDS_DEPTH < spill_depth) spill[sp - LDS_DEPTH] = make_uint2((uint32_t)r.y, 0u);
-          else { *overflow = true; return Hit{-1.0f, -1}; }
-          sp++;
-          next = r.x;
-        } else if (hl) next = r.x;
-        else if (hr) next = r.y;
-      } else {
-        float tl = slab_tnear(llo, lhi, o, d, inv);
-        float tr = slab_tnear(rlo, rhi, o, d, inv);
-        float thr = has ? best * 1.001f : INF;
-        bool vl = hl && tl <= thr, vr = hr && tr <= thr;
-        if (vl && vr) {
-          int32_t nearRef = (tr < tl) ? r.y : r.x;
-          int32_t farRef = (tr < tl) ? r.x : r.y;
-          float farT = (tr < tl) ? tl : tr;
-          uint2 e = make_uint2((uint32_t)farRef, __float_as_uint(farT));
-          if (sp < LDS_DEPTH) lds_stack[sp * stride] = e;
-          else if (sp - LDS_DEPTH < spill_depth) spill[sp - LDS_DEPTH] = e;
-          else { *overflow = true; return Hit{-1.0f, -1}; }
-          sp++;
-          next = nearRef;
-        } else if (vl) next = r.x;
-        else if (vr) next = r.y;
-      }
-      if (next == INT32_MIN) { ref = INT32_MIN; break; }
-      ref = next;
-    }
-    if (ref != INT32_MIN) {  // a leaf
-      const int32_t leaf = ~ref;
-      const int32_t p0 = T.leaf_start[leaf], p1 = T.leaf_start[leaf + 1];
-      for (int32_t p = p0; p < p1; p++) {
-        if (STATS) st->tris++;
-        float4 a = T.tv0[p], b = T.tv1[p], c = T.tv2[p];
-        float dist = tri_dist(xyz(a), xyz(b), xyz(c), o, d);
-        if (!(dist >= 0)) continue;
-        bool win = !has || dist < best || (dist == best && leaf > bestLeaf);
-        if (!win) continue;
-        if (LEGALITY) {
-          float4 n = T.tn[p];
-          if (!legal(d, xyz(n), __float_as_int(b.w), inside, contact)) continue;
+    const unsigned long long idle = __ballot(!active);
+    if (idle && !drained) {
+      const int leader = __ffsll((long long)idle) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(blk_next, (uint32_t)__popcll(idle));
+      base = __shfl(base, leader);
+      if (base + (uint32_t)__popcll(idle) >= hi) drained = true;
+      if (!active) {
+        const uint32_t i = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        if (i < hi) {
+          cur = i;
+          Vec3 o, d;
+          int32_t ins = -1, con = -1;
+          fetch(i, o, d, ins, con);
+          active = trace_begin<STATS>(T, ts, o, d, ins, con, &st);
+          if (!active) emit(i, Hit{-1.0f, -1}, st);  // missed the scene box
         }
-        has = true;
-        best = dist;
-        bestLeaf = leaf;
-        bestPos = p;
       }
     }
-    // pop
-    ref = INT32_MIN;
-    while (sp > 0) {
-      sp--;
-      uint2 e = (sp < LDS_DEPTH) ? lds_stack[sp * stride] : spill[sp - LDS_DEPTH];
-      if (EXHAUSTIVE || !has || __uint_as_float(e.y) <= best * 1.001f) {
-        ref = (int32_t)e.x;
-        break;
-      }
+    if (drained && !__any(active)) break;
+    if (active &&
+        trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH>(T, ts, lds_stack, spill, spill_depth, &st, overflow)) {
+      emit(cur, ts.has ? Hit{ts.best, ts.bestPos} : Hit{-1.0f, -1}, st);
+      active = false;
     }
-    if (ref == INT32_MIN) break;
   }
-  if (has) {
-    h.dist = best;
-    h.pos = bestPos;
-  }
-  return h;
 }
 
 // ------------------------------------------------------------------ RNG

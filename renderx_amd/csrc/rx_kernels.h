@@ -8,7 +8,10 @@
 namespace rx {
 
 constexpr int kTraceBlock = 128;   // threads per traversal block (2 waves)
-constexpr int kLdsDepth = 16;      // LDS stack entries per lane
+#ifndef RX_LDS_DEPTH
+#define RX_LDS_DEPTH 12
+#endif
+constexpr int kLdsDepth = RX_LDS_DEPTH;  // LDS stack entries per lane
 
 struct IsectArgs {
   uint32_t n;
@@ -25,8 +28,11 @@ struct IsectArgs {
   int32_t* overflow;   // set to 1 on stack overflow
 };
 
-// Grid-stride nearest-hit over a batch; grid_blocks bounds the spill buffer.
+// Nearest hit over a batch in grid_blocks contiguous ranges (grid_blocks
+// also sizes the spill buffer).
 hipError_t launch_intersect(const DevTree& T, const IsectArgs& a, uint32_t flags, int grid_blocks,
                             hipStream_t s);
+// Resident blocks per CU of the batch traversal kernel (LDS / registers).
+int intersect_blocks_per_cu();
 
 }  // namespace rx

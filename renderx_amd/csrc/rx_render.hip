@@ -263,29 +263,42 @@ __global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C, Path
 }
 
 // ------------------------------------------------------------------ extend
+// Nearest hit of every queued path.  Block k runs the contiguous queue range
+// [k*per, (k+1)*per) with trace_range (lanes refill from the block's LDS
+// counter as their rays end); every ray's result is independent of the lane
+// that ran it.
 template <bool LEG, bool STATS>
 __global__ void __launch_bounds__(kRBlock) k_extend(DevTree T, PathBufs B, uint32_t b) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
+  __shared__ uint32_t blk_next;
   const uint32_t n = B.cnt[b];
   const uint32_t* q = (b & 1) ? B.q1 : B.q0;
+  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t lo = min(n, blockIdx.x * per), hi = min(n, lo + per);
+  if (threadIdx.x == 0) blk_next = lo;
+  __syncthreads();
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  uint2* spill = B.spill + (size_t)gtid * B.spill_depth;
   uint32_t boxes = 0, tris = 0;
   bool ovf = false;
-  for (uint32_t i = gtid; i < n; i += stride) {
-    const uint32_t p = q[i];
-    float4 o = B.o[p], d = B.d[p];
-    int4 s = B.s[p];
-    TraceStats st;
-    Hit h = trace<false, LEG, STATS, kLdsDepth>(T, xyz(o), xyz(d), s.x, s.y, stack + threadIdx.x, spill,
-                                                B.spill_depth, &st, &ovf);
-    B.hit[p] = make_int2(__float_as_int(h.dist), h.pos);
-    if (STATS) {
-      boxes += st.boxes;
-      tris += st.tris;
-    }
-  }
+  trace_range<false, LEG, STATS, kLdsDepth>(
+      T, hi, &blk_next, stack + threadIdx.x, B.spill + (size_t)gtid * B.spill_depth, B.spill_depth, &ovf,
+      [&](uint32_t i, Vec3& o, Vec3& d, int32_t& ins, int32_t& con) {
+        const uint32_t p = q[i];
+        o = xyz(B.o[p]);
+        d = xyz(B.d[p]);
+        if (LEG) {
+          const int4 s = B.s[p];
+          ins = s.x;
+          con = s.y;
+        }
+      },
+      [&](uint32_t i, Hit h, const TraceStats& st) {
+        B.hit[q[i]] = make_int2(__float_as_int(h.dist), h.pos);
+        if (STATS) {
+          boxes += st.boxes;
+          tris += st.tris;
+        }
+      });
   if (ovf) *B.overflow = 1;
   if (STATS) {
     wave_add(&B.stats[4], boxes);
@@ -1246,7 +1259,14 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   RXCK(R.mean.ensure(12 * (size_t)npix), "alloc");
   RXCK(R.cnt.ensure(4 * kCntWords), "alloc");
   RXCK(R.stats.ensure(8 * 8), "alloc");
-  const int blocks = (int)std::min<uint64_t>((N + kRBlock - 1) / kRBlock, (uint64_t)c->n_cu * 8);
+  if (!c->render_blocks_per_cu) {  // launches are sized to be resident at once (see k_extend)
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<true, false>, kRBlock, 0) != hipSuccess || nb < 1)
+      nb = 4;
+    c->render_blocks_per_cu = nb;
+  }
+  const int blocks =
+      (int)std::min<uint64_t>((N + kRBlock - 1) / kRBlock, (uint64_t)c->n_cu * c->render_blocks_per_cu);
   int sd = 0;
   if (int r = ensure_spill(c, blocks, kRBlock, (int)R.scene.max_depth, &sd)) return r;
   PathBufs B;

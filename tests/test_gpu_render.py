@@ -57,7 +57,8 @@ def _render(rx, ctx, scene, spp, mode, seed=11, **kw):
 CASES = {
     "cornell": (lambda: scenes.cornell(RES, RES, spp=16, max_depth=8, rr_start_depth=3), 16384),
     "media": (lambda: scenes.media_scene(RES, RES, spp=16, max_depth=16, freq=24, glass_freq=8), 32768),
-    "icosphere": (lambda: scenes.icosphere_scene(RES, RES, spp=16, max_depth=10, freq=24), 32768),
+    # a grey scene: the three channels of a pixel fail together, so more pixels
+    "icosphere": (lambda: scenes.icosphere_scene(32, 32, spp=16, max_depth=10, freq=24), 32768),
 }
 _oracle_cache = {}
 _gpu_pt_cache = {}
@@ -93,7 +94,7 @@ def test_image_matches_oracle_pathtracer(rx, oracle_mod, gpu_ctx, name, mode):
         _check(f"{name} mode {mode}", r["sum"], r["sumsq"], GPU_SPP, osum, osq, on, ovar=ovar,
                extra=(pt["sum"], pt["sumsq"]))
     st = r["stats"]
-    assert st["samples"] == RES * RES * GPU_SPP and st["rays"] >= st["extend_rays"] > 0
+    assert st["samples"] == osum.shape[0] * osum.shape[1] * GPU_SPP and st["rays"] >= st["extend_rays"] > 0
 
 
 def test_cornell_nee_matches_pathtracertest(rx, oracle_mod, gpu_ctx):
