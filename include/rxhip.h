@@ -85,9 +85,21 @@ typedef struct rx_render_params {
 
 enum rx_render_flags {
   RX_RENDER_TIMING = 1,     /* time each kernel with HIP events (ms_extend/ms_shade/ms_shadow) */
-  RX_RENDER_COUNT_WORK = 2, /* count box and triangle tests of the extend kernels */
-  RX_RENDER_NO_GRAPH = 4    /* launch kernels one by one instead of replaying a captured pass graph */
+  RX_RENDER_COUNT_WORK = 2  /* count box and triangle tests of the extend kernels */
 };
+
+/* Launch shape of rx_render; no counterpart in the reference.  Every setting
+ * gives bit-identical images (the film adds each pixel's samples in sample
+ * order and the random numbers are keyed by pixel and sample); only the time
+ * changes.  Zero fields take the defaults. */
+typedef struct rx_render_tuning {
+  uint32_t path_slots;    /* path states resident at once (default 2^22): a pass renders
+                             min(spp, path_slots / pixels) samples of every pixel (>= 1) */
+  uint32_t blocks_per_cu; /* resident 128-thread blocks per CU of the render kernels
+                             (default and maximum: the traversal kernel's occupancy limit) */
+  uint32_t use_graph;     /* 1: record one pass into a hipGraph and replay it for every pass */
+  uint32_t reserved;
+} rx_render_tuning;
 
 typedef struct rx_render_stats {
   uint64_t rays;           /* all traversal queries: camera, bounce, shadow, containment */
@@ -188,6 +200,59 @@ int rx_load_scene(rx_ctx* ctx, uint32_t n_verts, const float* verts, uint32_t n_
  * deterministic: identical outputs for identical inputs. */
 int rx_render(rx_ctx* ctx, const rx_camera_desc* camera, const rx_render_params* params, float* rgb_mean,
               double* sum, double* sumsq, rx_render_stats* stats);
+
+/* Sets / reads the context's rx_render_tuning (validated; zero fields = defaults). */
+int rx_set_render_tuning(rx_ctx* ctx, const rx_render_tuning* tuning);
+int rx_get_render_tuning(rx_ctx* ctx, rx_render_tuning* tuning);
+/* Times rx_render with each candidate launch shape on `params` (spp capped
+ * at params->spp) and keeps the fastest in the context; `best` (may be NULL)
+ * receives it and `best_ms` its time.  No counterpart in the reference. */
+int rx_autotune_render(rx_ctx* ctx, const rx_camera_desc* camera, const rx_render_params* params,
+                       rx_render_tuning* best, float* best_ms);
+
+/* ------------------------------------------------------------------ multi-scene batch */
+/* No counterpart in the reference, which renders one scene per run
+ * (src/Renderer.cpp:50): many scenes resident together, rendered as one
+ * wavefront job whose passes mix the scenes' paths (a pass holds at most one
+ * sample of any pixel; with rx_render_tuning.use_graph one pass is recorded
+ * into a hipGraph and replayed).  Each scene's film is bit-identical to
+ * rx_load_scene + rx_render of that scene with the same camera and params.
+ * Scenes with media (RX_OBJ_VPM) are rejected; all scenes share one mode
+ * (RX_MODE_PT or RX_MODE_NEE). */
+typedef struct rx_batch rx_batch;
+int rx_batch_create(rx_ctx* ctx, rx_batch** out);
+void rx_batch_destroy(rx_batch* batch);
+/* Builds the scene's tree (as rx_load_scene) and queues it with its camera
+ * and render params; *index (may be NULL) receives its position. */
+int rx_batch_add_scene(rx_batch* batch, uint32_t n_verts, const float* verts, uint32_t n_tris, const uint32_t* tris,
+                       uint32_t n_objects, const rx_object_desc* objects, uint32_t tree_flags,
+                       const rx_camera_desc* camera, const rx_render_params* params, uint32_t* index);
+/* Renders every queued scene; films stay on the device. */
+int rx_batch_render(rx_batch* batch, rx_render_stats* stats);
+/* One scene's film as rx_render returns it (any output may be NULL). */
+int rx_batch_get_image(rx_batch* batch, uint32_t index, float* rgb_mean, double* sum, double* sumsq);
+
+/* ------------------------------------------------------------------ film I/O and comparison (host only) */
+/* Layout: RGB float32 row-major y*W+x, y = 0 the top row (rx_render's output). */
+/* MCRenderer::saveImagePFM (src/MCRenderer.cpp:556-574): "PF\n<W> <H>\n-1.000000\n"
+ * header, rows from the bottom of the image up, R G B float32 little endian. */
+int rx_save_pfm(const char* path, int32_t width, int32_t height, const float* rgb);
+/* The time-stamped copy of MCRenderer::showCurrentResult (src/MCRenderer.cpp:465-490):
+ * writes save_path with its extension replaced by "_time_<time>_iter_<iter>.pfm". */
+int rx_save_pfm_snapshot(const char* save_path, uint32_t time, uint32_t iter, int32_t width, int32_t height,
+                         const float* rgb);
+/* readImagePFM (include/visualizeUtils.h:23-48): rgb may be NULL to query the
+ * size; clamp != 0 clamps channels to [0, 4] like the reference.  The header
+ * is parsed per the PFM format (scale sign = byte order), unlike the
+ * reference's %d parse of the scale. */
+int rx_read_pfm(const char* path, int32_t* width, int32_t* height, float* rgb, int32_t clamp);
+/* CalculateRMSE (cmp/compare.cpp:85-124): sqrt(mean over masked pixels of
+ * |ref - img|^2); mask (W*H bytes, non-zero = counted) may be NULL. */
+int rx_image_rmse(int32_t width, int32_t height, const float* img, const float* ref, const uint8_t* mask,
+                  double* rmse);
+/* No counterpart in the reference: relMSE = mean over pixels and channels of
+ * (img - ref)^2 / (ref^2 + eps), the path-guiding convention (eps = 1e-2). */
+int rx_image_relmse(int32_t width, int32_t height, const float* img, const float* ref, float eps, double* relmse);
 
 /* ------------------------------------------------------------------ path guiding */
 typedef struct rx_guide_params {

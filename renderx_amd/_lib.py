@@ -38,6 +38,19 @@ SIGNATURES = {
     "rx_copy_kernel": (C.c_int, [P, P, P, U64]),
     "rx_load_scene": (C.c_int, [P, U32, P, U32, P, U32, P, U32]),
     "rx_render": (C.c_int, [P, P, P, P, P, P, P]),
+    "rx_set_render_tuning": (C.c_int, [P, P]),
+    "rx_get_render_tuning": (C.c_int, [P, P]),
+    "rx_autotune_render": (C.c_int, [P, P, P, P, P]),
+    "rx_batch_create": (C.c_int, [P, C.POINTER(P)]),
+    "rx_batch_destroy": (None, [P]),
+    "rx_batch_add_scene": (C.c_int, [P, U32, P, U32, P, U32, P, U32, P, P, P]),
+    "rx_batch_render": (C.c_int, [P, P]),
+    "rx_batch_get_image": (C.c_int, [P, U32, P, P, P]),
+    "rx_save_pfm": (C.c_int, [C.c_char_p, I32, I32, P]),
+    "rx_save_pfm_snapshot": (C.c_int, [C.c_char_p, U32, U32, I32, I32, P]),
+    "rx_read_pfm": (C.c_int, [C.c_char_p, C.POINTER(I32), C.POINTER(I32), P, I32]),
+    "rx_image_rmse": (C.c_int, [I32, I32, P, P, P, C.POINTER(C.c_double)]),
+    "rx_image_relmse": (C.c_int, [I32, I32, P, P, F32, C.POINTER(C.c_double)]),
     "rx_guide_init": (C.c_int, [P, P]),
     "rx_guide_set_bounds": (C.c_int, [P, P, P]),
     "rx_guide_set_weights": (C.c_int, [P, P]),

@@ -15,7 +15,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import check, lib, ptr
+from ._lib import RxError, check, lib, ptr  # noqa: F401  (RxError re-exported)
 
 ORDERED, EXHAUSTIVE, LEGALITY = 0, 1, 2
 
@@ -25,7 +25,8 @@ RENDER_STATS = np.dtype([("rays", "<u8"), ("extend_rays", "<u8"), ("shadow_rays"
                          ("samples", "<u8"), ("box_tests", "<u8"), ("tri_tests", "<u8"), ("ms_total", "<f8"),
                          ("ms_extend", "<f8"), ("ms_shade", "<f8"), ("ms_shadow", "<f8"),
                          ("extend_launches", "<u8")])
-RENDER_TIMING, RENDER_COUNT_WORK, RENDER_NO_GRAPH = 1, 2, 4
+RENDER_TIMING, RENDER_COUNT_WORK = 1, 2
+RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use_graph", "<u4"), ("reserved", "<u4")])
 GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("batch_size", "<u4"), ("steps_per_pass", "<u4"), ("learning_rate", "<f4"),
                          ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4")])
 MODE_PT, MODE_NEE, MODE_GUIDED = 0, 1, 2
@@ -144,6 +145,30 @@ class PathTracerHIP:
         p["seed"] = seed
         self.ctx.check(lib().rx_guide_init(self.ctx.h, ptr(p)), "rx_guide_init")
 
+    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0):
+        """Launch shape of renderPixels (rx_render_tuning); images do not depend on it."""
+        t = np.zeros(1, RENDER_TUNING)
+        t["path_slots"], t["blocks_per_cu"], t["use_graph"] = path_slots, blocks_per_cu, use_graph
+        self.ctx.check(lib().rx_set_render_tuning(self.ctx.h, ptr(t)), "rx_set_render_tuning")
+
+    def get_tuning(self):
+        t = np.zeros(1, RENDER_TUNING)
+        self.ctx.check(lib().rx_get_render_tuning(self.ctx.h, ptr(t)), "rx_get_render_tuning")
+        return {k: int(t[k][0]) for k in RENDER_TUNING.names if k != "reserved"}
+
+    def autotune(self, spp=None, max_depth=None, rr_start_depth=None, mode=MODE_PT, seed=1):
+        """rx_autotune_render: time the candidate launch shapes, keep the fastest."""
+        cam = np.ascontiguousarray(self.scene.camera.reshape(1))
+        p = np.zeros(1, RENDER_PARAMS)
+        p["spp"] = self.scene.spp if spp is None else spp
+        p["max_depth"] = self.scene.max_depth if max_depth is None else max_depth
+        p["rr_start_depth"] = self.scene.rr_start_depth if rr_start_depth is None else rr_start_depth
+        p["mode"], p["seed"] = mode, seed
+        t = np.zeros(1, RENDER_TUNING)
+        ms = np.zeros(1, np.float32)
+        self.ctx.check(lib().rx_autotune_render(self.ctx.h, ptr(cam), ptr(p), ptr(t), ptr(ms)), "rx_autotune_render")
+        return {k: int(t[k][0]) for k in RENDER_TUNING.names if k != "reserved"}, float(ms[0])
+
     def renderPixels(self, camera=None, spp=None, max_depth=None, rr_start_depth=None, mode=MODE_PT, seed=1,
                      sample_offset=0, want_sums=True, flags=0):
         """Returns dict(mean=(H,W,3) f32, sum, sumsq=(H,W,3) f64, stats)."""
@@ -164,3 +189,51 @@ class PathTracerHIP:
         self.ctx.check(lib().rx_render(self.ctx.h, ptr(cam), ptr(p), ptr(mean), ptr(s), ptr(q), ptr(st)),
                        "rx_render")
         return {"mean": mean, "sum": s, "sumsq": q, "stats": {k: st[k][0].item() for k in RENDER_STATS.names}}
+
+
+class BatchRenderer:
+    """Many scenes rendered as one wavefront job (rx_batch_*): every scene's
+    image is identical to PathTracerHIP.renderPixels of that scene alone."""
+
+    def __init__(self, device: int = 0, ctx: Context | None = None):
+        self.ctx = ctx or Context(device)
+        h = C.c_void_p()
+        self.ctx.check(lib().rx_batch_create(self.ctx.h, C.byref(h)), "rx_batch_create")
+        self.h = h
+        self.shapes = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().rx_batch_destroy(self.h)
+            self.h = None
+
+    def add(self, scene, spp=None, max_depth=None, rr_start_depth=None, mode=MODE_PT, seed=1, sample_offset=0,
+            camera=None, tree_abs_int=False) -> int:
+        v = np.ascontiguousarray(scene.verts, np.float32)
+        t = np.ascontiguousarray(scene.tris, np.uint32)
+        o = np.ascontiguousarray(scene.objects)
+        cam = np.ascontiguousarray((scene.camera if camera is None else camera).reshape(1))
+        p = np.zeros(1, RENDER_PARAMS)
+        p["spp"] = scene.spp if spp is None else spp
+        p["max_depth"] = scene.max_depth if max_depth is None else max_depth
+        p["rr_start_depth"] = scene.rr_start_depth if rr_start_depth is None else rr_start_depth
+        p["mode"], p["seed"], p["sample_offset"] = mode, seed, sample_offset
+        idx = C.c_uint32()
+        self.ctx.check(lib().rx_batch_add_scene(self.h, len(v), ptr(v), len(t), ptr(t), len(o), ptr(o),
+                                                1 if tree_abs_int else 0, ptr(cam), ptr(p), C.byref(idx)),
+                       "rx_batch_add_scene")
+        self.shapes.append((int(cam["height"][0]), int(cam["width"][0])))
+        return int(idx.value)
+
+    def render(self):
+        st = np.zeros(1, RENDER_STATS)
+        self.ctx.check(lib().rx_batch_render(self.h, ptr(st)), "rx_batch_render")
+        return {k: st[k][0].item() for k in RENDER_STATS.names}
+
+    def image(self, k):
+        H, W = self.shapes[k]
+        mean = np.empty((H, W, 3), np.float32)
+        s = np.empty((H, W, 3), np.float64)
+        q = np.empty((H, W, 3), np.float64)
+        self.ctx.check(lib().rx_batch_get_image(self.h, k, ptr(mean), ptr(s), ptr(q)), "rx_batch_get_image")
+        return {"mean": mean, "sum": s, "sumsq": q}

@@ -17,8 +17,9 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librxhip.so")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["rx_capi.hip", "rx_traverse.hip", "rx_render.hip", "rx_guide.hip"]
-CXX_SOURCES = ["rx_tree.cpp"]
+HIP_SOURCES = ["rx_capi.hip", "rx_traverse.hip", "rx_render.hip", "rx_batch.hip", "rx_guide.hip"]
+CXX_SOURCES = ["rx_tree.cpp", "rx_film.cpp"]
+CMP_OUT = os.path.join(HERE, "rx_cmp")  # the cmp tool (host only)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -53,6 +54,10 @@ def build(verbose: bool = False, out: str = OUT, defines: tuple = ()) -> str:
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, out)
+    if not defines:
+        _run(["g++", *CXX_FLAGS, "-o", CMP_OUT + ".tmp", os.path.join(CSRC, "rx_cmp.cpp"),
+              os.path.join(build_dir, "rx_film.cpp.o")])
+        os.replace(CMP_OUT + ".tmp", CMP_OUT)
     return out
 
 

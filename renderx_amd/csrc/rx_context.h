@@ -65,6 +65,7 @@ struct rx_ctx {
   int device = 0;
   int n_cu = 256;
   int isect_blocks_per_cu = 0, render_blocks_per_cu = 0;  // resident traversal blocks (0: not queried yet)
+  rx_render_tuning tuning{};                               // rx_set_render_tuning (zero = defaults)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;

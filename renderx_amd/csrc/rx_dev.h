@@ -197,9 +197,18 @@ struct TraceState {
   int sp;
 };
 
+// Root of one tree inside a node array shared by several trees.
+struct TreeRoot {
+  int32_t ref;
+  float lo[3], hi[3];
+};
+__device__ __forceinline__ TreeRoot tree_root(const DevTree& T) {
+  return TreeRoot{T.root_ref, {T.root_lo[0], T.root_lo[1], T.root_lo[2]}, {T.root_hi[0], T.root_hi[1], T.root_hi[2]}};
+}
+
 // Root box test and set-up; false when the ray misses the scene box.
 template <bool STATS>
-__device__ __forceinline__ bool trace_begin(const DevTree& T, TraceState& s, Vec3 o, Vec3 d, int32_t inside,
+__device__ __forceinline__ bool trace_begin(const TreeRoot& R, TraceState& s, Vec3 o, Vec3 d, int32_t inside,
                                             int32_t contact, TraceStats* st) {
   if (STATS) { st->boxes = 1; st->tris = 0; }
   s.o = o;
@@ -210,11 +219,11 @@ __device__ __forceinline__ bool trace_begin(const DevTree& T, TraceState& s, Vec
   s.best = -1.0f;
   s.bestLeaf = s.bestPos = -1;
   s.sp = 0;
-  s.ref = T.root_ref;
-  if (!box_hit(T.root_lo, T.root_hi, o, d)) return false;
+  s.ref = R.ref;
+  if (!box_hit(R.lo, R.hi, o, d)) return false;
   s.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   s.beta = v3(0, 0, 0);
-  s.fast = slab_filter_setup(T.root_lo, T.root_hi, d, s.inv, s.beta);
+  s.fast = slab_filter_setup(R.lo, R.hi, d, s.inv, s.beta);
   return true;
 }
 
@@ -324,7 +333,7 @@ __device__ __forceinline__ Hit trace(const DevTree& T, Vec3 o, Vec3 d, int32_t i
                                      uint2* lds_stack, uint2* spill, int spill_depth, TraceStats* st,
                                      bool* overflow) {
   TraceState s;
-  if (!trace_begin<STATS>(T, s, o, d, inside, contact, st)) return Hit{-1.0f, -1};
+  if (!trace_begin<STATS>(tree_root(T), s, o, d, inside, contact, st)) return Hit{-1.0f, -1};
   while (!trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH>(T, s, lds_stack, spill, spill_depth, st, overflow)) {
   }
   return s.has ? Hit{s.best, s.bestPos} : Hit{-1.0f, -1};
@@ -334,8 +343,9 @@ __device__ __forceinline__ Hit trace(const DevTree& T, Vec3 o, Vec3 d, int32_t i
 // advances it a trip at a time; lanes whose ray has ended take the next
 // entry through the block's LDS counter *blk_next (initialised to lo by the
 // caller before a barrier), so a long ray holds up only its own lane.
-// fetch(i, o, d, inside, contact) loads entry i; emit(i, hit, stats) stores
-// its result.  Every wave leaves once the range is drained and its lanes
+// fetch(i, o, d, inside, contact, root) loads entry i (root starts as T's
+// own and may name another tree of the node array); emit(i, hit, stats)
+// stores its result.  Every wave leaves once the range is drained and its lanes
 // are idle.
 template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, class Fetch, class Emit>
 __device__ __forceinline__ void trace_range(const DevTree& T, uint32_t hi, uint32_t* blk_next, uint2* lds_stack,
@@ -359,8 +369,9 @@ __device__ __forceinline__ void trace_range(const DevTree& T, uint32_t hi, uint3
           cur = i;
           Vec3 o, d;
           int32_t ins = -1, con = -1;
-          fetch(i, o, d, ins, con);
-          active = trace_begin<STATS>(T, ts, o, d, ins, con, &st);
+          TreeRoot root = tree_root(T);
+          fetch(i, o, d, ins, con, root);
+          active = trace_begin<STATS>(root, ts, o, d, ins, con, &st);
           if (!active) emit(i, Hit{-1.0f, -1}, st);  // missed the scene box
         }
       }

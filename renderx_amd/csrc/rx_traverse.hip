@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(DevTree T, IsectArgs 
   bool ovf = false;
   trace_range<EXH, LEG, STATS, kLdsDepth>(
       T, hi, &blk_next, stack + threadIdx.x, a.spill + (size_t)gtid * a.spill_depth, a.spill_depth, &ovf,
-      [&](uint32_t i, Vec3& o, Vec3& d, int32_t& ins, int32_t& con) {
+      [&](uint32_t i, Vec3& o, Vec3& d, int32_t& ins, int32_t& con, TreeRoot&) {
         o = v3(a.org[3 * (size_t)i], a.org[3 * (size_t)i + 1], a.org[3 * (size_t)i + 2]);
         d = v3(a.dir[3 * (size_t)i], a.dir[3 * (size_t)i + 1], a.dir[3 * (size_t)i + 2]);
         if (LEG) {

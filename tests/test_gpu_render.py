@@ -156,3 +156,25 @@ def test_furnace_sphere_closed_form(rx, gpu_ctx, mode):
     print(f"  f={f:.4f} expect {expect:.6f} got {got:.6f} (se {se:.2e})")
     # tessellation makes the sphere identity approximate: 1 % on top of the noise
     assert abs(got - expect) < 0.01 * expect + 5 * se
+
+
+def test_launch_tuning_gives_identical_images(rx, gpu_ctx):
+    """rx_render_tuning (samples per pass, resident blocks) and the autotuner
+    change only the time: the film is bit-identical."""
+    print("[tuning]", flush=True)
+    sc = scenes.media_scene(40, 40, freq=16, glass_freq=6)
+    pt = rx.PathTracerHIP(sc, ctx=gpu_ctx)
+    pt.set_tuning()
+    a = pt.renderPixels(spp=48, mode=MODE_NEE, seed=3)
+    try:
+        for slots, blk in [(4096, 0), (40 * 40 * 5, 2), (1 << 20, 3)]:
+            pt.set_tuning(path_slots=slots, blocks_per_cu=blk)
+            b = pt.renderPixels(spp=48, mode=MODE_NEE, seed=3)
+            assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sumsq"], b["sumsq"]), (slots, blk)
+        best, ms = pt.autotune(spp=16, mode=MODE_NEE)
+        print(f"  autotuned {best} in {ms:.2f} ms")
+        assert best["path_slots"] > 0 and pt.get_tuning() == best
+        c = pt.renderPixels(spp=48, mode=MODE_NEE, seed=3)
+        assert np.array_equal(a["sum"], c["sum"])
+    finally:
+        pt.set_tuning()
