@@ -7,8 +7,9 @@ every bounce), FP32.  Inputs are generated synthetically and resident in HBM
 before the timed region; the tree is built once outside it.
 
 Prints ONE JSON line with the metric, the extend (traversal) kernel's roofline
-figures and the CPU baseline (the oracle's restated PathTracer on this host's
-cores over a bounded sample of the same scene).
+figures, the equal-time relMSE of the plain and the guided renderer on the
+media + glass scene, and the CPU baseline (the oracle's restated PathTracer on
+this host's cores over a bounded sample of the same scene).
 """
 from __future__ import annotations
 
@@ -34,7 +35,7 @@ B_TRI = 40
 # --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on the
 # rx_copy_kernel 256 MiB copy; the commands and raw figures are in
 # profiles/k_extend_icosphere1M_1024px.md
-EXTEND_TRAFFIC_BYTES = 59.05e6
+EXTEND_TRAFFIC_BYTES = 504.26e6
 
 
 def log(*a):
@@ -78,6 +79,39 @@ def cpu_baseline(scene, seconds_target=15.0):
                       f"{scene.max_depth} ({st['rays']} traversal queries in {dt:.1f} s)"}
 
 
+def equal_time_relmse(res=256, ref_spp=8192, budget_s=5.0, chunk=16):
+    """The metric's second half on the media + glass scene (homogeneous
+    medium box, glass sphere, Phong displaced sphere in a Cornell box): relMSE
+    against a converged image (NEE, ref_spp samples) of the plain path tracer
+    and the guided renderer (training included) after the same wall time."""
+    from renderx_amd import api, scenes, film
+
+    sc = scenes.media_scene(res, res, max_depth=16, freq=158, glass_freq=16)
+    ctx = api.Context(0)
+    t = time.perf_counter()
+    ref = api.PathTracerHIP(sc, ctx=ctx).renderPixels(spp=ref_spp, mode=api.MODE_NEE, seed=99)["mean"]
+    ref_s = time.perf_counter() - t
+    log(f"[bench] relMSE reference ({ref_spp} spp NEE) in {ref_s:.1f} s")
+    out = {"scene": f"media+glass+Phong sphere {sc.n_tris} tris, {res}x{res} px, max depth 16",
+           "reference": f"NEE {ref_spp} spp", "budget_s": budget_s}
+    for name, mode in [("plain", api.MODE_PT), ("nee", api.MODE_NEE), ("guided", api.MODE_GUIDED)]:
+        pt = api.PathTracerHIP(sc, ctx=ctx)
+        if mode == api.MODE_GUIDED:
+            pt.guide_init(train_passes=16, batch_size=4096, steps_per_pass=16, learning_rate=1e-2, seed=7)
+        acc, n = None, 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s:
+            r = pt.renderPixels(spp=chunk, mode=mode, seed=5, sample_offset=n)
+            acc = r["sum"] if acc is None else acc + r["sum"]
+            n += chunk
+        img = (acc / n).astype(np.float32)
+        out[name] = film.relmse(img, ref)
+        out[name + "_spp"] = n
+        log(f"[bench] {name}: {n} spp in {time.perf_counter() - t0:.1f} s, relMSE {out[name]:.5f}")
+    out["plain_over_guided"] = out["plain"] / out["guided"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +120,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-relmse", action="store_true", help="skip the equal-time relMSE study")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -163,6 +198,8 @@ def main():
         b = torch.empty_like(a)
         pt.ctx.check(_lib.lib().rx_copy_kernel(pt.ctx.h, b.data_ptr(), a.data_ptr(), nbytes), "rx_copy_kernel")
         log(f"[bench] calibration: rx::k_copy16 over {args.calib_copy_mib} MiB done")
+    if not args.no_relmse:
+        out["equal_time_relmse"] = equal_time_relmse()
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(scene)

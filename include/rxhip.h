@@ -254,6 +254,28 @@ int rx_image_rmse(int32_t width, int32_t height, const float* img, const float* 
  * (img - ref)^2 / (ref^2 + eps), the path-guiding convention (eps = 1e-2). */
 int rx_image_relmse(int32_t width, int32_t height, const float* img, const float* ref, float eps, double* relmse);
 
+/* ------------------------------------------------------------------ scene files (host only) */
+/* ConfigManager::load (src/ConfigManager.cpp:166-405) for this path: the
+ * Config XML, the Scene XML it names (GroupObject / Object / Material /
+ * Camera nodes, findNode's Name matching, paths relative to the config's
+ * folder), OBJ and SHP shapes (SimpleShape::loadShape, src/SimpleShape.cpp:44-313:
+ * quads split in two, GroupObjects split by "g" groups and unitized), object
+ * transforms applied like getWorldVertexPosition (include/SimpleShape.h:44).
+ * Materials: Diffuse, Glossy, Phong, Refractive, Emissive, VPM; renderer
+ * PathTracer/PT (RX_MODE_PT) or PTtest (RX_MODE_NEE).  Textures, other
+ * materials and other renderers are rejected with a message in err. */
+typedef struct rx_scene_file rx_scene_file;
+int rx_scene_load(const char* config_path, rx_scene_file** out, char* err, uint32_t err_cap);
+void rx_scene_file_free(rx_scene_file* file);
+int rx_scene_file_sizes(const rx_scene_file* file, uint32_t* n_verts, uint32_t* n_tris, uint32_t* n_objects);
+/* Copies the arrays rx_load_scene takes (3*n_verts floats, 3*n_tris indices,
+ * n_objects descriptors), the camera, the render params (spp from
+ * <Renderer><spp>, default 1; maxDepth default 20), savePath and the <time>
+ * budget in seconds.  Any output may be NULL. */
+int rx_scene_file_get(const rx_scene_file* file, float* verts, uint32_t* tris, rx_object_desc* objects,
+                      rx_camera_desc* camera, rx_render_params* params, char* save_path, uint32_t save_cap,
+                      double* runtime_s);
+
 /* ------------------------------------------------------------------ path guiding */
 typedef struct rx_guide_params {
   uint32_t train_passes;   /* leading samples per pixel that record training data (one pass each) */

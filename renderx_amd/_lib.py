@@ -51,6 +51,10 @@ SIGNATURES = {
     "rx_read_pfm": (C.c_int, [C.c_char_p, C.POINTER(I32), C.POINTER(I32), P, I32]),
     "rx_image_rmse": (C.c_int, [I32, I32, P, P, P, C.POINTER(C.c_double)]),
     "rx_image_relmse": (C.c_int, [I32, I32, P, P, F32, C.POINTER(C.c_double)]),
+    "rx_scene_load": (C.c_int, [C.c_char_p, C.POINTER(P), C.c_char_p, U32]),
+    "rx_scene_file_free": (None, [P]),
+    "rx_scene_file_sizes": (C.c_int, [P, P, P, P]),
+    "rx_scene_file_get": (C.c_int, [P, P, P, P, P, P, C.c_char_p, U32, C.POINTER(C.c_double)]),
     "rx_guide_init": (C.c_int, [P, P]),
     "rx_guide_set_bounds": (C.c_int, [P, P, P]),
     "rx_guide_set_weights": (C.c_int, [P, P]),

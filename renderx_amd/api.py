@@ -237,3 +237,35 @@ class BatchRenderer:
         q = np.empty((H, W, 3), np.float64)
         self.ctx.check(lib().rx_batch_get_image(self.h, k, ptr(mean), ptr(s), ptr(q)), "rx_batch_get_image")
         return {"mean": mean, "sum": s, "sumsq": q}
+
+
+def load_scene_config(config_path: str):
+    """ConfigManager::load for this path (rx_scene_load): the Config XML, its
+    Scene XML and the OBJ/SHP shapes.  Returns a scenes.Scene (spp, max depth
+    from the file) with meta {"mode", "save_path", "runtime_s"}."""
+    import os
+
+    from . import scenes
+
+    h = C.c_void_p()
+    err = C.create_string_buffer(1024)
+    if lib().rx_scene_load(os.fsencode(config_path), C.byref(h), err, len(err)) != 0:
+        raise RxError(f"rx_scene_load: {err.value.decode(errors='replace')}")
+    try:
+        nv, nt, no = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        lib().rx_scene_file_sizes(h, C.byref(nv), C.byref(nt), C.byref(no))
+        verts = np.empty((nv.value, 3), np.float32)
+        tris = np.empty((nt.value, 3), np.uint32)
+        objs = np.zeros(no.value, scenes.OBJ_DTYPE)
+        cam = np.zeros((), scenes.CAM_DTYPE)
+        prm = np.zeros(1, RENDER_PARAMS)
+        save = C.create_string_buffer(4096)
+        rt = C.c_double()
+        check(None, lib().rx_scene_file_get(h, ptr(verts), ptr(tris), ptr(objs), cam.ctypes.data_as(C.c_void_p),
+                                            ptr(prm), save, len(save), C.byref(rt)), "rx_scene_file_get")
+    finally:
+        lib().rx_scene_file_free(h)
+    name = os.path.splitext(os.path.basename(config_path))[0]
+    return scenes.Scene(name, verts, tris, objs, cam, int(prm["spp"][0]), int(prm["max_depth"][0]), 0,
+                        {"mode": int(prm["mode"][0]), "save_path": save.value.decode(errors="replace"),
+                         "runtime_s": rt.value})

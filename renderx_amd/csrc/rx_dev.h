@@ -193,6 +193,9 @@ struct TraceState {
   Vec3 o, d, inv, beta;
   bool fast, has;
   float best;
+  float limit;  // subtrees whose entry key exceeds it are culled (INF: none)
+  float anyhit; // with use_anyhit, a legal hit with dist - anyhit < -1e-5 ends the traversal
+  bool use_anyhit;
   int32_t bestLeaf, bestPos, ref, inside, contact;
   int sp;
 };
@@ -217,6 +220,9 @@ __device__ __forceinline__ bool trace_begin(const TreeRoot& R, TraceState& s, Ve
   s.contact = contact;
   s.has = false;
   s.best = -1.0f;
+  s.limit = __builtin_inff();
+  s.anyhit = 0.0f;
+  s.use_anyhit = false;
   s.bestLeaf = s.bestPos = -1;
   s.sp = 0;
   s.ref = R.ref;
@@ -276,7 +282,7 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
       } else if (hl) next = r.x;
       else if (hr) next = r.y;
     } else {
-      const float thr = s.has ? s.best * 1.001f : INF;
+      const float thr = fminf(s.has ? s.best * 1.001f : INF, s.limit);
       const bool vl = hl && tl <= thr, vr = hr && tr <= thr;
       if (vl && vr) {
         const int32_t nearRef = (tr < tl) ? r.y : r.x;
@@ -311,6 +317,10 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
       s.best = dist;
       s.bestLeaf = leaf;
       s.bestPos = p;
+      if (s.use_anyhit && dist - s.anyhit < -1e-5f) {  // the caller only needs to know that such a hit exists
+        s.ref = INT32_MIN;
+        return true;
+      }
     }
   }
   // pop
@@ -318,7 +328,8 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
   while (s.sp > 0) {
     s.sp--;
     const uint2 e = (s.sp < LDS_DEPTH) ? lds_stack[s.sp * stride] : spill[s.sp - LDS_DEPTH];
-    if (EXHAUSTIVE || !s.has || __uint_as_float(e.y) <= s.best * 1.001f) {
+    const float key = __uint_as_float(e.y);
+    if (EXHAUSTIVE || (s.has ? key <= fminf(s.best * 1.001f, s.limit) : !(key > s.limit))) {
       ref = (int32_t)e.x;
       break;
     }

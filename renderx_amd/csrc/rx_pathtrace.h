@@ -913,61 +913,111 @@ template <bool BATCH>
 __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
                                                     BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
+  __shared__ uint32_t blk_next;
   const uint32_t n = B.cnt[kShadowBase + b];
+  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t lo = min(n, blockIdx.x * per), hi = min(n, lo + per);
+  if (threadIdx.x == 0) blk_next = lo;
+  __syncthreads();
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t stride = gridDim.x * blockDim.x;
   uint2* lds = stack + threadIdx.x;
   uint2* spill = B.spill + (size_t)gtid * B.spill_depth;
+  const int lane = threadIdx.x & 63;
   uint32_t rays = 0, srays = 0;
-  bool ovf = false;
-  for (uint32_t i = gtid; i < n; i += stride) {
-    const uint32_t p = B.sq[i];
-    DevScene S = S0;
-    const uint32_t max_depth = path_ctx<BATCH>(P, BA, p, S).max_depth;
-    const float4 so = B.so[p], sd = B.sd[p], scv = B.sc[p];
-    int2 ctx = B.ss[p];
-    Vec3 o = xyz(so), w = xyz(sd);
-    float remaining = so.w;
-    int inside = ctx.x, contact = ctx.y;
-    Vec3 T = v3(1, 1, 1);
-    bool visible = false;
-    for (int c = 0; c < kMaxCross; c++) {
-      TraceStats st;
-      Hit h = trace<false, true, false, kLdsDepth>(S.tree, o, w, inside, contact, lds, spill, B.spill_depth, &st,
-                                                   &ovf);
-      rays++;
-      srays++;
-      if (h.pos < 0 || h.dist - remaining >= -1e-5f) {
-        visible = b + 1 + (uint32_t)c <= max_depth;
-        break;
-      }
-      const int hobj = __float_as_int(S.tree.tv1[h.pos].w);
-      if (S.objs[hobj].type != OBJ_VPM) break;
-      Vec3 xn = o + w * h.dist;
-      int next_inside = (inside != hobj) ? hobj : find_inside(S, xn, w, hobj, lds, B, rays);
-      if (refr_coeff(S, inside) != refr_coeff(S, next_inside)) break;  // would bend: not a straight shadow ray
-      if (inside != NONE && S.objs[inside].type == OBJ_VPM) {
-        const DevObject& M = S.objs[inside];
-        Vec3 tr = vexp(v3(-M.st[0] * h.dist, -M.st[1] * h.dist, -M.st[2] * h.dist));
-        T = T * tr * (__expf(-M.lum_st * h.dist) / lum(tr));
-      }
-      o = xn;
-      remaining -= h.dist;
-      inside = next_inside;
-      contact = hobj;
-    }
+  bool ovf = false, active = false, drained = false;
+  // the lane's query: from o along w to the light point at distance `remaining`
+  uint32_t p = 0, max_depth = 0;
+  int c = 0, inside = NONE, contact = NONE;
+  Vec3 o = v3(0, 0, 0), w = v3(0, 0, 0), T = v3(1, 1, 1);
+  float remaining = 0.0f;
+  DevScene S = S0;
+  TraceState ts;
+  TraceStats st{0u, 0u};
+  auto finish = [&](bool visible) {
     if (visible && inside != NONE && S.objs[inside].type == OBJ_VPM) {
       const DevObject& M = S.objs[inside];
       Vec3 tr = vexp(v3(-M.st[0] * remaining, -M.st[1] * remaining, -M.st[2] * remaining));
       T = T * tr * (__expf(-M.lum_st * remaining) / lum(tr));
     }
     if (visible) {
-      float4 L = B.L[p];
+      const float4 scv = B.sc[p];
+      const float4 L = B.L[p];
       const Vec3 add = v3(scv.x * T.x, scv.y * T.y, scv.z * T.z);
       B.L[p] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.0f);
       if (GA.tag > 0.0f) {  // the vertex's own NEE is not part of the radiance along its sampled direction
         float* r = GA.rec + ((size_t)p * GA.R + b) * kRecFloats;
         if (r[15] == GA.tag) r[14] += lum(add);
+      }
+    }
+    return false;
+  };
+  // trace the next straight segment; false when the query is already decided
+  auto start = [&]() {
+    rays++;
+    srays++;
+    if (!trace_begin<false>(tree_root(S.tree), ts, o, w, inside, contact, &st))
+      return finish(b + 1 + (uint32_t)c <= max_depth);  // no hit at all
+    ts.limit = remaining * 1.001f;  // hits beyond the light point cannot block it
+    ts.anyhit = remaining;
+    ts.use_anyhit = S.n_vol == 0;  // without media the first blocking hit decides
+    return true;
+  };
+  for (;;) {
+    const unsigned long long idle = __ballot(!active);
+    if (idle && !drained) {
+      const int leader = __ffsll((long long)idle) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&blk_next, (uint32_t)__popcll(idle));
+      base = __shfl(base, leader);
+      if (base + (uint32_t)__popcll(idle) >= hi) drained = true;
+      if (!active) {
+        const uint32_t i = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        if (i < hi) {
+          p = B.sq[i];
+          if (BATCH) S = S0;  // path_ctx rebinds the scene's root and emitters
+          max_depth = path_ctx<BATCH>(P, BA, p, S).max_depth;
+          const float4 so = B.so[p], sd = B.sd[p];
+          const int2 ctx = B.ss[p];
+          o = xyz(so);
+          w = xyz(sd);
+          remaining = so.w;
+          inside = ctx.x;
+          contact = ctx.y;
+          T = v3(1, 1, 1);
+          c = 0;
+          active = start();
+        }
+      }
+    }
+    if (drained && !__any(active)) break;
+    if (active && trace_trip<false, true, false, kLdsDepth>(S.tree, ts, lds, spill, B.spill_depth, &st, &ovf)) {
+      // MCRenderer::testVisibility: visible when the first legal hit is not nearer than the light point
+      if (!ts.has || ts.best - remaining >= -1e-5f) {
+        active = finish(b + 1 + (uint32_t)c <= max_depth);
+      } else {
+        const float hd = ts.best;
+        const int hobj = __float_as_int(S.tree.tv1[ts.bestPos].w);
+        if (S.objs[hobj].type != OBJ_VPM) {
+          active = finish(false);
+        } else {  // cross an index-matched medium boundary and go on
+          const Vec3 xn = o + w * hd;
+          const int next_inside = (inside != hobj) ? hobj : find_inside(S, xn, w, hobj, lds, B, rays);
+          if (refr_coeff(S, inside) != refr_coeff(S, next_inside)) {  // would bend: not a straight shadow ray
+            active = finish(false);
+          } else {
+            if (inside != NONE && S.objs[inside].type == OBJ_VPM) {
+              const DevObject& M = S.objs[inside];
+              Vec3 tr = vexp(v3(-M.st[0] * hd, -M.st[1] * hd, -M.st[2] * hd));
+              T = T * tr * (__expf(-M.lum_st * hd) / lum(tr));
+            }
+            o = xn;
+            remaining -= hd;
+            inside = next_inside;
+            contact = hobj;
+            c++;
+            active = c < kMaxCross ? start() : finish(false);
+          }
+        }
       }
     }
   }

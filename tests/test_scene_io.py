@@ -1,0 +1,172 @@
+"""Scene ingestion (rx_scene_load, renderx_amd/csrc/rx_scene_io.cpp): the
+ConfigManager::load subset of this path (src/ConfigManager.cpp:166-405) with
+SimpleShape::loadShape (src/SimpleShape.cpp:44-313) and the XML helpers
+(src/xmlHelper.cpp).  Checked on files written here: exact round trip of a
+generated scene, transforms in the reference's float order, GroupObject
+splitting and unitize, Name matching, materials, camera and renderer nodes.
+No GPU needed."""
+import os
+
+import numpy as np
+import pytest
+
+api = pytest.importorskip("renderx_amd.api")
+from renderx_amd import scenes  # noqa: E402
+
+TYPE_NAMES = {scenes.DIFFUSE: "Diffuse", scenes.PHONG: "Phong", scenes.EMISSIVE: "Emissive",
+              scenes.REFRACTIVE: "Refractive", scenes.VPM: "VPM"}
+
+
+def _vec(v):
+    return "(" + ", ".join(f"{float(x):.9g}" for x in v) + ")"
+
+
+def _material_xml(o):
+    t = int(o["type"])
+    if t == scenes.DIFFUSE:
+        return f"<type>Diffuse</type><color>{_vec(o['color'])}</color>"
+    if t == scenes.PHONG:
+        return (f"<type>Phong</type><GlossyComponent><color>{_vec(o['color'])}</color><coeff>{float(o['exponent']):.9g}"
+                f"</coeff></GlossyComponent><DiffuseComponent><color>{_vec(o['color2'])}</color></DiffuseComponent>")
+    if t == scenes.EMISSIVE:
+        return f"<type>Emissive</type><color>{_vec(o['color'])}</color>"
+    if t == scenes.REFRACTIVE:
+        return (f"<type>Refractive</type><coeff>{float(o['ior']):.9g}</coeff><surfColor>{_vec(o['color'])}</surfColor>"
+                f"<decayColor>{_vec(o['color2'])}</decayColor>")
+    if t == scenes.VPM:
+        return (f"<type>VPM</type><sigma_s>{_vec(o['color'])}</sigma_s><sigma_a>{_vec(o['color2'])}</sigma_a>"
+                f"<inrate>{float(o['ior']):.9g}</inrate><g>{float(o['g']):.9g}</g>")
+    raise ValueError(t)
+
+
+def _write_scene(tmp, sc, renderer="PT", spp=32, max_depth=9):
+    """One OBJ per object (its own vertices), identity transforms."""
+    objs_xml = []
+    for k, o in enumerate(sc.objects):
+        t0, n = int(o["tri_begin"]), int(o["tri_count"])
+        tri = sc.tris[t0:t0 + n]
+        used, inv = np.unique(tri.ravel(), return_inverse=True)
+        with open(tmp / f"obj{k}.obj", "w") as f:
+            f.write("# generated\n")
+            for v in sc.verts[used]:
+                f.write(f"v {float(v[0]):.9g} {float(v[1]):.9g} {float(v[2]):.9g}\n")
+            for a, b, c in inv.reshape(-1, 3) + 1:
+                f.write(f"f {a} {b} {c}\n")
+        objs_xml.append(f'<Object Name="o{k}"><filePath>obj{k}.obj</filePath>'
+                        f"<Material>{_material_xml(o)}</Material></Object>")
+    cam = sc.camera
+    (tmp / "scene.xml").write_text(
+        '<?xml version="1.0"?>\n<!-- a scene -->\n<Scene Name="main">\n' + "\n".join(objs_xml) +
+        f'\n<Camera Name="c1"><width>{int(cam["width"])}</width><height>{int(cam["height"])}</height>'
+        f'<sightDist>{int(cam["sight_dist"])}</sightDist><focus>{_vec(cam["focus"])}</focus>'
+        f'<position>{_vec(cam["position"])}</position><up>{_vec(cam["up"])}</up></Camera>\n</Scene>\n')
+    (tmp / "config.xml").write_text(
+        "<Config>\n <time><minutes>2</minutes></time>\n <Scene><path>scene.xml</path><name>main</name>"
+        f"<camera>c1</camera></Scene>\n <Renderer><type>{renderer}</type><spp>{spp}</spp><maxDepth>{max_depth}"
+        "</maxDepth></Renderer>\n <savePath>results/out.pfm</savePath>\n</Config>\n")
+    return str(tmp / "config.xml")
+
+
+def _world(sc):
+    """Per-triangle world vertex triples (the renderer only sees these)."""
+    return sc.verts[sc.tris]
+
+
+def test_round_trip_of_a_generated_scene(tmp_path):
+    sc = scenes.media_scene(24, 20, freq=6, glass_freq=3)
+    cfg = _write_scene(tmp_path, sc)
+    ld = api.load_scene_config(cfg)
+    assert np.array_equal(_world(ld), _world(sc))
+    for k in range(len(sc.objects)):
+        a, b = ld.objects[k], sc.objects[k]
+        for f in ("type", "tri_count", "color", "exponent", "color2", "ior", "g"):
+            if int(b["type"]) in (scenes.DIFFUSE, scenes.EMISSIVE) and f in ("color2", "exponent", "ior", "g"):
+                continue
+            if int(b["type"]) == scenes.PHONG and f in ("ior", "g"):
+                continue
+            assert np.array_equal(a[f], b[f]), (k, f)
+    for f in ("width", "height", "sight_dist", "position", "focus", "up"):
+        assert np.array_equal(ld.camera[f], sc.camera[f]), f
+    assert ld.spp == 32 and ld.max_depth == 9 and ld.meta["mode"] == api.MODE_PT
+    assert ld.meta["save_path"] == "results/out.pfm" and ld.meta["runtime_s"] == 120.0
+
+
+def test_transforms_quads_groups_and_name_matching(tmp_path):
+    m = np.array([[0.5, 0.0, 0.25, 1.5], [0.0, 2.0, 0.0, -0.75], [0.1, 0.0, 1.0, 0.3], [0, 0, 0, 1]], np.float32)
+    mtxt = "".join("[" + ", ".join(f"{x:.9g}" for x in row) + "]" for row in m)
+    (tmp_path / "quad.obj").write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3//1 4//1\n")
+    # a group file: two named groups, unitized as a whole
+    (tmp_path / "groups.obj").write_text(
+        "v 0 0 0\nv 4 0 0\nv 0 2 0\ng first\nv 10 0 0\nv 12 0 1\nv 10 2 0\nf 4 5 6\ng second\n"
+        "v 6 6 6\nv 7 6 6\nv 6 7 6\nf 7 8 9\n")
+    (tmp_path / "mats.xml").write_text('<Material Name="red"><type>Diffuse</type><color>(0.9, 0.1, 0.1)</color>'
+                                       '</Material><Material Name="lamp"><type>Emissive</type>'
+                                       "<color>(5, 5, 5)</color></Material>")
+    (tmp_path / "scene.xml").write_text(
+        '<Scene Name="other"><Camera Name="x"/></Scene>'
+        '<Scene Name="s"><Material Name="local"><type>Glossy</type><color>(0.5, 0.5, 0.5)</color>'
+        '<coeff>20</coeff></Material>'
+        f'<GroupObject><filePath>groups.obj</filePath><transform>{mtxt}</transform>'
+        '<Object Name="second"><Material><path>mats.xml</path><name>lamp</name></Material></Object>'
+        '<Object Name="first"><Material><name>local</name></Material></Object></GroupObject>'
+        f'<Object><filePath>quad.obj</filePath><transform>{mtxt}</transform>'
+        '<Material><path>mats.xml</path><name>red</name></Material></Object>'
+        '<Camera Name="wrong"><width>1</width></Camera>'
+        '<Camera Name="cam"><width>8</width><height>6</height><sightDist>9</sightDist><focus>(0, 0, 0)</focus>'
+        '<position>(0, 0, 5)</position><up>(0, 1, 0)</up></Camera></Scene>')
+    (tmp_path / "config.xml").write_text("<Config><Scene><path>scene.xml</path><name>s</name><camera>cam</camera>"
+                                         "</Scene><renderer>PTtest</renderer><maxDepth>5</maxDepth></Config>")
+    ld = api.load_scene_config(str(tmp_path / "config.xml"))
+    t = [int(o["type"]) for o in ld.objects]
+    assert t == [scenes.EMISSIVE, scenes.GLOSSY, scenes.DIFFUSE]
+    assert ld.objects[1]["exponent"] == 20.0 and np.allclose(ld.objects[2]["color"], [0.9, 0.1, 0.1])
+
+    def xform(p):  # getWorldVertexPosition's float order
+        p = np.asarray(p, np.float32)
+        out = np.zeros(3, np.float32)
+        for r in range(3):
+            x = np.float32(p[0] * m[r, 0])
+            x = np.float32(x + np.float32(p[1] * m[r, 1]))
+            x = np.float32(x + np.float32(p[2] * m[r, 2]))
+            out[r] = np.float32(x + m[r, 3])
+        return out
+
+    # unitize over all 9 file vertices: box [0,12]x[0,7]x[0,6], centre (6, 3.5, 3), half size 6
+    def unit(p):
+        c = np.array([6.0, 3.5, 3.0], np.float32)
+        s = np.float32(2) / np.float32(12)
+        return [np.float32(np.float32(p[a] * s) + np.float32(-np.float32(2) / np.float32(12) * c[a])) for a in range(3)]
+
+    w = _world(ld)
+    assert np.array_equal(w[0], np.array([xform(unit(p)) for p in [(6, 6, 6), (7, 6, 6), (6, 7, 6)]]))
+    assert np.array_equal(w[1], np.array([xform(unit(p)) for p in [(10, 0, 0), (12, 0, 1), (10, 2, 0)]]))
+    quad = [xform(p) for p in [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)]]
+    assert np.array_equal(w[2], np.array([quad[0], quad[1], quad[2]]))
+    assert np.array_equal(w[3], np.array([quad[0], quad[2], quad[3]]))
+    assert ld.camera["width"] == 8 and ld.camera["sight_dist"] == 9
+    assert ld.meta["mode"] == api.MODE_NEE and ld.max_depth == 5 and ld.spp == 1
+
+
+@pytest.mark.parametrize("material,msg", [
+    ("<type>Diffuse</type><Texture><file>a.png</file></Texture>", "texture"),
+    ("<type>GridVolume</type>", "not on this path"),
+    ("<type>Glossy</type><color>(1, 1, 1)</color>", "coeff"),
+])
+def test_rejections(tmp_path, material, msg):
+    (tmp_path / "t.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    (tmp_path / "config.xml").write_text(
+        f"<Config><Scene><camera>c</camera></Scene></Config><Scene><Object><filePath>t.obj</filePath>"
+        f"<Material>{material}</Material></Object><Camera><width>2</width><height>2</height><sightDist>2</sightDist>"
+        "<focus>(0, 0, 0)</focus><position>(0, 0, 1)</position><up>(0, 1, 0)</up></Camera></Scene>")
+    with pytest.raises(api.RxError, match=msg):
+        api.load_scene_config(str(tmp_path / "config.xml"))
+
+
+def test_unknown_renderer_and_missing_file(tmp_path):
+    with pytest.raises(api.RxError):
+        api.load_scene_config(str(tmp_path / "nope.xml"))
+    sc = scenes.cornell(8, 8)
+    cfg = _write_scene(tmp_path, sc, renderer="VCM")
+    with pytest.raises(api.RxError, match="VCM"):
+        api.load_scene_config(cfg)
+    assert os.path.exists(cfg)
