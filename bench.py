@@ -8,7 +8,7 @@ before the timed region; the tree is built once outside it.
 
 Prints ONE JSON line with the metric, the extend (traversal) kernel's roofline
 figures, the equal-time relMSE of the plain and the guided renderer on the
-media + glass scene, and the CPU baseline (the oracle's restated PathTracer on
+media + glass scene, the throughput of the 64-scene batch, and the CPU baseline (the oracle's restated PathTracer on
 this host's cores over a bounded sample of the same scene).
 """
 from __future__ import annotations
@@ -112,6 +112,37 @@ def equal_time_relmse(res=256, ref_spp=8192, budget_s=5.0, chunk=16):
     return out
 
 
+def batch64(steps=2):
+    """The 64-scene triangle-soup batch (10^4..2*10^6 triangles per scene,
+    128^2..1024^2 pixels, 4..64 spp, max depth 8): all scenes rendered as one
+    job with one recorded pass graph replayed per pass.  Returns rays/s."""
+    from renderx_amd import api, scenes
+
+    t = time.perf_counter()
+    scs = [scenes.soup_scene(i) for i in range(64)]
+    log(f"[bench] batch64: generated {sum(s.n_tris for s in scs)} triangles in {time.perf_counter() - t:.1f} s")
+    ctx = api.Context(0)
+    br = api.BatchRenderer(ctx=ctx)
+    t = time.perf_counter()
+    for k, sc in enumerate(scs):
+        br.add(sc, mode=api.MODE_PT, seed=1000 + k)
+    build_s = time.perf_counter() - t
+    log(f"[bench] batch64: 64 trees built in {build_s:.1f} s")
+    br.set_tuning(use_graph=1)
+    br.render()  # warm-up
+    rays = samples = 0
+    t = time.perf_counter()
+    for _ in range(steps):
+        st = br.render()
+        rays += st["rays"]
+        samples += st["samples"]
+    wall = time.perf_counter() - t
+    log(f"[bench] batch64: {samples / steps / 1e6:.0f} M samples per batch, {wall / steps * 1e3:.0f} ms per batch")
+    return {"scenes": 64, "triangles": int(sum(s.n_tris for s in scs)), "samples_per_batch": samples // steps,
+            "ms_per_batch": wall / steps * 1e3, "rays_per_s": rays / wall, "samples_per_s": samples / wall,
+            "host_tree_build_s": build_s}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +152,7 @@ def main():
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-relmse", action="store_true", help="skip the equal-time relMSE study")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -200,6 +232,8 @@ def main():
         log(f"[bench] calibration: rx::k_copy16 over {args.calib_copy_mib} MiB done")
     if not args.no_relmse:
         out["equal_time_relmse"] = equal_time_relmse()
+    if not args.no_batch:
+        out["batch64"] = batch64()
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(scene)

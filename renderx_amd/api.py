@@ -225,6 +225,12 @@ class BatchRenderer:
         self.shapes.append((int(cam["height"][0]), int(cam["width"][0])))
         return int(idx.value)
 
+    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0):
+        """Launch shape (rx_render_tuning of the context); images do not depend on it."""
+        t = np.zeros(1, RENDER_TUNING)
+        t["path_slots"], t["blocks_per_cu"], t["use_graph"] = path_slots, blocks_per_cu, use_graph
+        self.ctx.check(lib().rx_set_render_tuning(self.ctx.h, ptr(t)), "rx_set_render_tuning")
+
     def render(self):
         st = np.zeros(1, RENDER_STATS)
         self.ctx.check(lib().rx_batch_render(self.h, ptr(st)), "rx_batch_render")

@@ -102,20 +102,24 @@ __host__ __device__ __forceinline__ float slab_tnear(const float lo[3], const fl
   return t;
 }
 
-// Certain-miss filter for box_hit.  If box_hit passes a face of axis c at
-// dist t, the real hit point t* = (plane - o_c) / d_c >= 0 lies in every
-// slab a widened by the rounding of the face check, |p_a - p_a*| <=
+// Certain-miss / certain-hit filter for box_hit.  If box_hit passes a face
+// of axis c at dist t, the real hit point t* = (plane - o_c) / d_c >= 0 lies
+// in every slab a widened by the rounding of the face check, |p_a - p_a*| <=
 // 4u (|t*| |d_a| + B_a) (u = 2^-24, B_a = max |coordinate| of the scene box):
 // in t units e_a - 4u t* - beta_a <= t* <= x_a + 4u t* + beta_a with
 // beta_a = 4u B_a / |d_a| (+ an absolute floor).  The slab ends e_a, x_a are
 // computed here with reciprocals (relative error <= 3u, bounded by 8u with
 // this function's own rounding), so when no t >= 0 satisfies the widened
-// bounds, box_hit is certainly false and may be skipped.  Needs
-// |d_a| >= 2^-30 on every axis (no overflow, no infinities); `beta` holds
-// 16u B_a / |d_a| + 2^-120 / |d_a|.  The key is slab_tnear's value.
-__host__ __device__ __forceinline__ bool slab_certain_miss(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
-                                                           Vec3 beta, float& tnear) {
-  const float U8 = 8.0f * 0x1p-24f;
+// bounds, box_hit is certainly false (kBoxMiss).  Conversely, if the ray
+// enters the box (or, from inside, leaves it) through the face of axis A at
+// t* > 0 and t* lies inside every other slab by more than those bounds, the
+// face check of that face passes and box_hit is certainly true (kBoxHit).
+// Needs |d_a| >= 2^-30 on every axis (no overflow, no infinities); `beta`
+// holds 16u B_a / |d_a| + 2^-120 / |d_a|.  tnear receives slab_tnear's value.
+constexpr int kBoxUnknown = 0, kBoxMiss = 1, kBoxHit = 2;
+__host__ __device__ __forceinline__ int slab_classify(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
+                                                      Vec3 beta, float& tnear) {
+  const float U8 = 8.0f * 0x1p-24f, U24 = 24.0f * 0x1p-24f;
   const float t0x = (lo[0] - o.x) * inv.x, t1x = (hi[0] - o.x) * inv.x;
   const float t0y = (lo[1] - o.y) * inv.y, t1y = (hi[1] - o.y) * inv.y;
   const float t0z = (lo[2] - o.z) * inv.z, t1z = (hi[2] - o.z) * inv.z;
@@ -126,12 +130,41 @@ __host__ __device__ __forceinline__ bool slab_certain_miss(const float lo[3], co
   t = ey > t ? ey : t;
   t = ez > t ? ez : t;
   tnear = t;
+  // widened slab ends: true e_a - beta_a >= le_a, true x_a + beta_a <= hx_a
   const float lx = fmaf(-U8, fabsf(ex), ex) - beta.x, hxx = fmaf(U8, fabsf(xx), xx) + beta.x;
   const float ly = fmaf(-U8, fabsf(ey), ey) - beta.y, hxy = fmaf(U8, fabsf(xy), xy) + beta.y;
   const float lz = fmaf(-U8, fabsf(ez), ez) - beta.z, hxz = fmaf(U8, fabsf(xz), xz) + beta.z;
   const float L = fmaxf(fmaxf(0.0f, lx), fmaxf(ly, lz));
   const float R = fminf(hxx, fminf(hxy, hxz));
-  return L * (1.0f - 16.0f * 0x1p-24f) > R * (1.0f + 16.0f * 0x1p-24f);
+  if (L * (1.0f - 16.0f * 0x1p-24f) > R * (1.0f + 16.0f * 0x1p-24f)) return kBoxMiss;
+  // candidate face: the entry face if the entry is clearly ahead, else the exit face
+  const bool entry = fmaf(-U8, fabsf(t), t) > 0.0f;
+  float tf = xx;
+  int af = 0;
+  if (entry) {
+    af = (t == ex) ? 0 : ((t == ey) ? 1 : 2);
+    tf = t;
+  } else {
+    tf = xx;
+    if (xy < tf) { tf = xy; af = 1; }
+    if (xz < tf) { tf = xz; af = 2; }
+    if (!(fmaf(-U8, fabsf(tf), tf) > 0.0f)) return kBoxUnknown;
+  }
+  // the face point must lie inside the other two slabs with the rounding bounds to spare
+  const float tlo = tf * (1.0f - U24), thi = tf * (1.0f + U24);
+  const float eu[3] = {fmaf(U8, fabsf(ex), ex) + beta.x, fmaf(U8, fabsf(ey), ey) + beta.y,
+                       fmaf(U8, fabsf(ez), ez) + beta.z};
+  const float xd[3] = {fmaf(-U8, fabsf(xx), xx) - beta.x, fmaf(-U8, fabsf(xy), xy) - beta.y,
+                       fmaf(-U8, fabsf(xz), xz) - beta.z};
+  bool ok = true;
+  for (int a = 0; a < 3; a++)
+    if (a != af) ok = ok && eu[a] <= tlo && thi <= xd[a];
+  return ok ? kBoxHit : kBoxUnknown;
+}
+// The miss half of slab_classify.
+__host__ __device__ __forceinline__ bool slab_certain_miss(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
+                                                           Vec3 beta, float& tnear) {
+  return slab_classify(lo, hi, o, inv, beta, tnear) == kBoxMiss;
 }
 // Per-ray set-up of slab_certain_miss; false when the filter must not be used.
 __host__ __device__ __forceinline__ bool slab_filter_setup(const float root_lo[3], const float root_hi[3], Vec3 d,
@@ -259,9 +292,11 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
     if (STATS) st->boxes += 2;
     bool hl, hr;
     float tl = 0.0f, tr = 0.0f;
-    if (s.fast) {  // the exact predicate only where the slab filter cannot rule the box out
-      hl = !slab_certain_miss(llo, lhi, s.o, s.inv, s.beta, tl) && box_hit(llo, lhi, s.o, s.d);
-      hr = !slab_certain_miss(rlo, rhi, s.o, s.inv, s.beta, tr) && box_hit(rlo, rhi, s.o, s.d);
+    if (s.fast) {  // the exact predicate only where the slab filter cannot decide
+      const int cl = slab_classify(llo, lhi, s.o, s.inv, s.beta, tl);
+      const int cr = slab_classify(rlo, rhi, s.o, s.inv, s.beta, tr);
+      hl = cl == kBoxHit || (cl == kBoxUnknown && box_hit(llo, lhi, s.o, s.d));
+      hr = cr == kBoxHit || (cr == kBoxUnknown && box_hit(rlo, rhi, s.o, s.d));
     } else {
       hl = box_hit(llo, lhi, s.o, s.d);
       hr = box_hit(rlo, rhi, s.o, s.d);

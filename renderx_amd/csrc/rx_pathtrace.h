@@ -24,7 +24,8 @@ constexpr int kRBlock = kTraceBlock;  // render kernels share the traversal bloc
 constexpr int kMaxBounce = 64;
 constexpr int kShadowBase = 96;       // cnt[kShadowBase + b]: shadow queue length of bounce b
 constexpr int kCntWords = 192;
-constexpr uint32_t kDefaultPathSlots = 1u << 22;  // rx_render_tuning.path_slots default
+constexpr uint32_t kDefaultPathSlots = 1u << 25;  // rx_render_tuning.path_slots default (~5.7 GB of path state)
+constexpr uint32_t kGuidedPathSlots = 1u << 22;   // guided mode: the network's per-row buffers scale with the slots
 constexpr int kMaxCross = 4;          // index-matched boundary crossings per shadow ray
 constexpr uint32_t FLAG_MIS = 1u;     // last scatter was RANDOM (only transparent crossings since)
 

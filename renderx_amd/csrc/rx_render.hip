@@ -173,7 +173,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   const uint32_t npix = (uint32_t)(cam->width * cam->height);
   // samples per pass: as many as the path-slot budget holds, so that the later
   // bounces (after Russian roulette) still fill the GPU
-  const uint32_t slots = c->tuning.path_slots ? c->tuning.path_slots : kDefaultPathSlots;
+  uint32_t slots = c->tuning.path_slots ? c->tuning.path_slots : kDefaultPathSlots;
+  if (prm->mode == RX_MODE_GUIDED) slots = std::min(slots, kGuidedPathSlots);
   const uint32_t S_max = std::max<uint32_t>(1u, std::min<uint32_t>(std::max(prm->spp, 1u), slots / npix));
   const uint32_t N = npix * S_max;  // path slots
 #define RXCK(x, w) \
@@ -267,11 +268,12 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       if (int r = guide_reserve_store(c, G, store_cap)) return r;
     const uint32_t Rs = prm->max_depth + 1;
     if (prm->sample_offset < train_passes && prm->spp) {
-      RXCK(R.rec.ensure((size_t)N * Rs * kRecFloats * 4), "alloc records");
-      RXCK(R.bcount.ensure(((size_t)N * Rs / kRecChunk + 2) * 4), "alloc records");
+      // a training pass renders one sample per pixel: npix paths
+      RXCK(R.rec.ensure((size_t)npix * Rs * kRecFloats * 4), "alloc records");
+      RXCK(R.bcount.ensure(((size_t)npix * Rs / kRecChunk + 2) * 4), "alloc records");
       RXCK(R.total.ensure(64), "alloc records");
       // tags restart at 1 every render: clear stale records once
-      hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, R.rec.as<double>(), (size_t)N * Rs * kRecFloats / 2);
+      hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, R.rec.as<double>(), (size_t)npix * Rs * kRecFloats / 2);
     }
     GA.net = G.net();
     GA.Z4 = G.Z4.as<float>();
@@ -406,8 +408,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
 
 int rx_set_render_tuning(rx_ctx* c, const rx_render_tuning* t) {
   if (!c || !t) return RX_ERR_INVALID;
-  if (t->path_slots && (t->path_slots < 4096 || t->path_slots > (1u << 26)))
-    return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: path_slots must be 0 or in [4096, 2^26]");
+  if (t->path_slots && (t->path_slots < 4096 || t->path_slots > (1u << 27)))
+    return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: path_slots must be 0 or in [4096, 2^27]");
   if (t->blocks_per_cu > 64) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: blocks_per_cu > 64");
   if (t->use_graph > 1) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: use_graph is 0 or 1");
   c->tuning = *t;
@@ -438,7 +440,7 @@ int rx_autotune_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_par
   rx_render_tuning win = keep;
   float win_ms = 0.0f;
   bool have = false;
-  const uint32_t slot_opts[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+  const uint32_t slot_opts[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
   const int blk_opts[] = {occ, std::max(2, (3 * occ) / 4)};
   for (uint32_t sl : slot_opts) {
     for (int bk : blk_opts) {

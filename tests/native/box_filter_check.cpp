@@ -1,6 +1,7 @@
-// CPU check of the traversal's certain-miss filter: over random and
-// adversarial (ray, box) pairs, slab_certain_miss may only claim a miss where
-// the reference box predicate (box_hit, src/KDTree.cpp:34-52) is false.
+// CPU check of the traversal's box filter: over random and adversarial
+// (ray, box) pairs, slab_classify may only claim a miss where the reference
+// box predicate (box_hit, src/KDTree.cpp:34-52) is false and a hit where it
+// is true.
 // Usage: box_filter_check <pairs per family> <seed>.  Prints one line of
 // counts per family; exits 1 on the first violation.
 #include <cmath>
@@ -28,7 +29,7 @@ int main(int argc, char** argv) {
   };
   long bad = 0;
   for (int fam = 0; fam < 7; fam++) {
-    long fast = 0, filtered = 0, exact_miss = 0, exact_hit = 0;
+    long fast = 0, filtered = 0, exact_miss = 0, exact_hit = 0, certain_hit = 0;
     for (long k = 0; k < n; k++) {
       const double R = std::pow(10.0, (int)(U(rng) * 5) - 1);  // scene half-size 0.1 .. 1000
       const float rlo[3] = {(float)-R, (float)-R, (float)-R}, rhi[3] = {(float)R, (float)R, (float)R};
@@ -99,7 +100,16 @@ int main(int argc, char** argv) {
       if (!slab_filter_setup(rlo, rhi, d, inv, beta)) continue;
       fast++;
       float tn;
-      const bool miss = slab_certain_miss(lo, hi, o, inv, beta, tn);
+      const int cls = slab_classify(lo, hi, o, inv, beta, tn);
+      const bool miss = cls == kBoxMiss;
+      if (cls == kBoxHit) {
+        certain_hit++;
+        if (!exact) {
+          printf("VIOLATION (certain hit) fam %d: lo %a %a %a hi %a %a %a o %a %a %a d %a %a %a\n", fam, lo[0], lo[1],
+                 lo[2], hi[0], hi[1], hi[2], o.x, o.y, o.z, d.x, d.y, d.z);
+          return 1;
+        }
+      }
       const float tref = slab_tnear(lo, hi, o, d, inv);
       if (!(tn == tref || (std::isnan(tn) && std::isnan(tref)))) {
         printf("key mismatch fam %d: %.9g vs %.9g\n", fam, tn, tref);
@@ -115,8 +125,8 @@ int main(int argc, char** argv) {
         }
       }
     }
-    printf("family %d: pairs %ld exact_hit %ld exact_miss %ld filter_usable %ld certain_miss %ld\n", fam, n, exact_hit,
-           exact_miss, fast, filtered);
+    printf("family %d: pairs %ld exact_hit %ld exact_miss %ld filter_usable %ld certain_miss %ld certain_hit %ld\n", fam,
+           n, exact_hit, exact_miss, fast, filtered, certain_hit);
   }
   return bad ? 1 : 0;
 }

@@ -1,9 +1,9 @@
-"""The traversal's certain-miss slab filter (slab_certain_miss in
-renderx_amd/csrc/rx_dev.h) never rules out a box that the reference box
-predicate (src/KDTree.cpp:34-52) accepts: checked on the CPU over random and
-adversarial rays (aimed at faces, edges and corners with ulp nudges, origins
-on face planes, near axis-parallel directions, flat boxes), and its ordering
-key equals slab_tnear's."""
+"""The traversal's slab filter (slab_classify in renderx_amd/csrc/rx_dev.h)
+never rules out a box that the reference box predicate (src/KDTree.cpp:34-52)
+accepts, and never certifies one it rejects: checked on the CPU over random
+and adversarial rays (aimed at faces, edges and corners with ulp nudges,
+origins on face planes, near axis-parallel directions, flat boxes); its
+ordering key equals slab_tnear's."""
 import os
 import subprocess
 
@@ -28,3 +28,6 @@ def test_certain_miss_filter_is_conservative(tmp_path):
     fam0 = lines[0].split()
     vals = {fam0[i]: int(fam0[i + 1]) for i in range(2, len(fam0) - 1, 2)}
     assert vals["certain_miss"] >= 0.99 * vals["exact_miss"]
+    fam6 = lines[6].split()
+    v6 = {fam6[i]: int(fam6[i + 1]) for i in range(2, len(fam6) - 1, 2)}
+    assert v6["certain_hit"] >= 0.3 * v6["exact_hit"]
