@@ -117,6 +117,9 @@ __host__ __device__ __forceinline__ float slab_tnear(const float lo[3], const fl
 // Needs |d_a| >= 2^-30 on every axis (no overflow, no infinities); `beta`
 // holds 16u B_a / |d_a| + 2^-120 / |d_a|.  tnear receives slab_tnear's value.
 constexpr int kBoxUnknown = 0, kBoxMiss = 1, kBoxHit = 2;
+#ifndef RX_CERTAIN_HIT
+#define RX_CERTAIN_HIT 1  // 0: only the miss half (fewer registers in the traversal loop)
+#endif
 __host__ __device__ __forceinline__ int slab_classify(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
                                                       Vec3 beta, float& tnear) {
   const float U8 = 8.0f * 0x1p-24f, U24 = 24.0f * 0x1p-24f;
@@ -137,6 +140,9 @@ __host__ __device__ __forceinline__ int slab_classify(const float lo[3], const f
   const float L = fmaxf(fmaxf(0.0f, lx), fmaxf(ly, lz));
   const float R = fminf(hxx, fminf(hxy, hxz));
   if (L * (1.0f - 16.0f * 0x1p-24f) > R * (1.0f + 16.0f * 0x1p-24f)) return kBoxMiss;
+#if RX_CERTAIN_HIT == 0
+  return kBoxUnknown;
+#endif
   // candidate face: the entry face if the entry is clearly ahead, else the exit face
   const bool entry = fmaf(-U8, fabsf(t), t) > 0.0f;
   float tf = xx;

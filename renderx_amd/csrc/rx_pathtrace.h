@@ -21,6 +21,11 @@ namespace rx {
 enum { OBJ_DIFFUSE = 0, OBJ_GLOSSY = 1, OBJ_PHONG = 2, OBJ_REFRACTIVE = 3, OBJ_EMISSIVE = 4, OBJ_VPM = 5 };
 constexpr int NONE = -1, CAMERA = -2;
 constexpr int kRBlock = kTraceBlock;  // render kernels share the traversal block shape
+#ifndef RX_EXTEND_MIN_WAVES
+#define RX_EXTEND_MIN_WAVES 1
+#endif
+// minimum resident waves per SIMD the extend kernel's registers must allow
+constexpr int kExtendMinWaves = RX_EXTEND_MIN_WAVES;
 constexpr int kMaxBounce = 64;
 constexpr int kShadowBase = 96;       // cnt[kShadowBase + b]: shadow queue length of bounce b
 constexpr int kCntWords = 192;
@@ -334,7 +339,7 @@ __global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C0, Pat
 // counter as their rays end); every ray's result is independent of the lane
 // that ran it.
 template <bool LEG, bool STATS, bool BATCH>
-__global__ void __launch_bounds__(kRBlock) k_extend(DevTree T, PathBufs B, uint32_t b, BatchArgs BA) {
+__global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend(DevTree T, PathBufs B, uint32_t b, BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
   __shared__ uint32_t blk_next;
   const uint32_t n = B.cnt[b];
