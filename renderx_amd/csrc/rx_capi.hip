@@ -268,7 +268,8 @@ int rx_intersect_dev(rx_ctx* c, uint32_t n, const float* org, const float* dir, 
   a.spill = c->spill.as<uint2>();
   a.spill_depth = sd;
   a.overflow = c->overflow.as<int32_t>();
-  if (int r = hip_err(c, hipMemsetAsync(a.overflow, 0, 4, c->stream), "overflow clear")) return r;
+  a.work = c->overflow.as<uint32_t>() + 1;
+  if (int r = hip_err(c, hipMemsetAsync(a.overflow, 0, 8, c->stream), "overflow clear")) return r;
   if (int r = hip_err(c, hipEventRecord(c->ev0, c->stream), "event")) return r;
   if (int r = hip_err(c, launch_intersect(c->tree.dev, a, flags, blocks, c->stream), "intersect launch")) return r;
   if (int r = hip_err(c, hipEventRecord(c->ev1, c->stream), "event")) return r;

@@ -391,44 +391,85 @@ __device__ __forceinline__ Hit trace(const DevTree& T, Vec3 o, Vec3 d, int32_t i
   return s.has ? Hit{s.best, s.bestPos} : Hit{-1.0f, -1};
 }
 
-// Runs the queue entries [lo, hi) of one block: each lane holds one ray and
-// advances it a trip at a time; lanes whose ray has ended take the next
-// entry through the block's LDS counter *blk_next (initialised to lo by the
-// caller before a barrier), so a long ray holds up only its own lane.
-// fetch(i, o, d, inside, contact, root) loads entry i (root starts as T's
-// own and may name another tree of the node array); emit(i, hit, stats)
-// stores its result.  Every wave leaves once the range is drained and its lanes
-// are idle.
-template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, class Fetch, class Emit>
-__device__ __forceinline__ void trace_range(const DevTree& T, uint32_t hi, uint32_t* blk_next, uint2* lds_stack,
-                                            uint2* spill, int spill_depth, bool* overflow, Fetch fetch, Emit emit) {
+#ifndef RX_CHUNK
+#define RX_CHUNK 256
+#endif
+constexpr uint32_t kChunk = RX_CHUNK;  // most queue entries a wave takes from the shared counter at a time
+
+// A wave's share of a queue [0, n): the waves take chunks of neighbouring
+// entries from a shared counter (zero at launch), and a wave hands its
+// chunk's entries to its idle lanes.  Neighbouring entries stay on one wave,
+// and the counter keeps every wave busy until the queue is drained, however
+// unevenly the costs fall over the queue.  Chunks hold about a quarter of an
+// even share of the queue per wave (at most kChunk), so that short queues
+// still spread over the whole grid.
+struct WaveWork {
+  uint32_t next = 0, end = 0;  // the wave's chunk [next, end), wave-uniform
+  uint32_t ch = 1;             // chunk size
+  bool drained = false;        // the counter has passed n
+};
+__device__ __forceinline__ WaveWork wave_work(uint32_t n) {
+  WaveWork w;
+  const uint32_t quarters = 4u * gridDim.x * (blockDim.x >> 6);
+  w.ch = max(1u, min(kChunk, (n + quarters - 1) / quarters));
+  return w;
+}
+// Called by every lane of a wave with the wave's ballot of idle lanes
+// (nonzero) while !w.drained; returns the entry for this lane if it is idle,
+// n if there is none.
+__device__ __forceinline__ uint32_t wave_take(WaveWork& w, unsigned long long idle, uint32_t n, uint32_t* work) {
   const int lane = threadIdx.x & 63;
-  bool active = false, drained = false;
+  const uint32_t need = (uint32_t)__popcll(idle);
+  const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+  const uint32_t avail = w.end - w.next;
+  uint32_t i = w.next + rank;
+  if (need > avail) {  // the chunk runs short: the next idle lanes take a new one
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(work, w.ch);
+    base = __shfl(base, 0);
+    if (rank >= avail) i = (rank - avail < w.ch) ? base + (rank - avail) : n;
+    if (base >= n) {
+      w.drained = true;
+      w.next = w.end = n;
+    } else {
+      w.end = min(n, base + w.ch);
+      w.next = min(w.end, base + (need - avail));
+    }
+  } else {
+    w.next += need;
+  }
+  return ((idle >> lane) & 1ull) && i < n ? i : n;
+}
+
+// Runs queue entries [0, n) over every wave of the grid (WaveWork): each lane
+// holds one ray and advances it a trip at a time, taking a new entry when its
+// ray has ended.  fetch(i, o, d, inside, contact, root) loads entry i (root
+// starts as T's own and may name another tree of the node array); emit(i,
+// hit, stats) stores its result.  A wave leaves once the queue is drained and
+// its lanes are idle.
+template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, class Fetch, class Emit>
+__device__ __forceinline__ void trace_range(const DevTree& T, uint32_t n, uint32_t* work, uint2* lds_stack,
+                                            uint2* spill, int spill_depth, bool* overflow, Fetch fetch, Emit emit) {
+  bool active = false;
   uint32_t cur = 0;
+  WaveWork w = wave_work(n);
   TraceState ts;
   TraceStats st{0u, 0u};
   for (;;) {
     const unsigned long long idle = __ballot(!active);
-    if (idle && !drained) {
-      const int leader = __ffsll((long long)idle) - 1;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(blk_next, (uint32_t)__popcll(idle));
-      base = __shfl(base, leader);
-      if (base + (uint32_t)__popcll(idle) >= hi) drained = true;
-      if (!active) {
-        const uint32_t i = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-        if (i < hi) {
-          cur = i;
-          Vec3 o, d;
-          int32_t ins = -1, con = -1;
-          TreeRoot root = tree_root(T);
-          fetch(i, o, d, ins, con, root);
-          active = trace_begin<STATS>(root, ts, o, d, ins, con, &st);
-          if (!active) emit(i, Hit{-1.0f, -1}, st);  // missed the scene box
-        }
+    if (idle && !w.drained) {
+      const uint32_t i = wave_take(w, idle, n, work);
+      if (i < n) {
+        cur = i;
+        Vec3 o, d;
+        int32_t ins = -1, con = -1;
+        TreeRoot root = tree_root(T);
+        fetch(i, o, d, ins, con, root);
+        active = trace_begin<STATS>(root, ts, o, d, ins, con, &st);
+        if (!active) emit(i, Hit{-1.0f, -1}, st);  // missed the scene box
       }
     }
-    if (drained && !__any(active)) break;
+    if (w.drained && !__any(active)) break;
     if (active &&
         trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH>(T, ts, lds_stack, spill, spill_depth, &st, overflow)) {
       emit(cur, ts.has ? Hit{ts.best, ts.bestPos} : Hit{-1.0f, -1}, st);

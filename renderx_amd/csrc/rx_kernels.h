@@ -9,7 +9,7 @@ namespace rx {
 
 constexpr int kTraceBlock = 128;   // threads per traversal block (2 waves)
 #ifndef RX_LDS_DEPTH
-#define RX_LDS_DEPTH 12
+#define RX_LDS_DEPTH 10
 #endif
 constexpr int kLdsDepth = RX_LDS_DEPTH;  // LDS stack entries per lane
 
@@ -26,6 +26,7 @@ struct IsectArgs {
   uint2* spill;        // grid_threads * spill_depth entries
   int spill_depth;     // tree depth beyond the LDS stack
   int32_t* overflow;   // set to 1 on stack overflow
+  uint32_t* work;      // shared work counter, zero at launch
 };
 
 // Nearest hit over a batch in grid_blocks contiguous ranges (grid_blocks

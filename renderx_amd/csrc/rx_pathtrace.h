@@ -22,13 +22,15 @@ enum { OBJ_DIFFUSE = 0, OBJ_GLOSSY = 1, OBJ_PHONG = 2, OBJ_REFRACTIVE = 3, OBJ_E
 constexpr int NONE = -1, CAMERA = -2;
 constexpr int kRBlock = kTraceBlock;  // render kernels share the traversal block shape
 #ifndef RX_EXTEND_MIN_WAVES
-#define RX_EXTEND_MIN_WAVES 1
+#define RX_EXTEND_MIN_WAVES 7
 #endif
 // minimum resident waves per SIMD the extend kernel's registers must allow
 constexpr int kExtendMinWaves = RX_EXTEND_MIN_WAVES;
 constexpr int kMaxBounce = 64;
 constexpr int kShadowBase = 96;       // cnt[kShadowBase + b]: shadow queue length of bounce b
-constexpr int kCntWords = 192;
+constexpr int kWorkBase = 192;       // cnt[kWorkBase + b]: extend work counter of bounce b
+constexpr int kShadowWorkBase = 256; // cnt[kShadowWorkBase + b]: shadow work counter of bounce b
+constexpr int kCntWords = 320;
 constexpr uint32_t kDefaultPathSlots = 1u << 25;  // rx_render_tuning.path_slots default (~5.7 GB of path state)
 constexpr uint32_t kGuidedPathSlots = 1u << 22;   // guided mode: the network's per-row buffers scale with the slots
 constexpr int kMaxCross = 4;          // index-matched boundary crossings per shadow ray
@@ -80,8 +82,30 @@ struct PathBufs {
 
 struct PassArgs {
   uint32_t n_paths, sample, key1, max_depth, rr_start, mode;
-  uint32_t npix;  // pixels; path slot p renders pixel p % npix, sample `sample + p / npix`
+  uint32_t npix;  // pixels
+  uint32_t spass; // samples of every pixel in this pass
 };
+
+#ifndef RX_PIXEL_MAJOR
+#define RX_PIXEL_MAJOR 1
+#endif
+// Path slot of (pixel q, j-th sample of the pass).  Pixel-major: the pass's
+// samples of one pixel occupy adjacent slots, so a wave's camera rays (and
+// the first bounces after them) start together and share most of their
+// traversal.  Slot order changes no result: the random streams are keyed by
+// (pixel, sample) and the film adds a pixel's samples in sample order.
+__host__ __device__ __forceinline__ uint32_t pass_slot(const PassArgs& P, uint32_t q, uint32_t j) {
+  return RX_PIXEL_MAJOR ? q * P.spass + j : j * P.npix + q;
+}
+__device__ __forceinline__ void slot_pixel(const PassArgs& P, uint32_t p, uint32_t& pix, uint32_t& smp) {
+  if (RX_PIXEL_MAJOR) {
+    pix = p / P.spass;
+    smp = P.sample + (p - pix * P.spass);
+  } else {
+    pix = p % P.npix;
+    smp = P.sample + p / P.npix;
+  }
+}
 
 // Multi-scene batch: the scenes' trees, objects and emitters are concatenated
 // (node, leaf, triangle and object indices rebased), so one DevScene holds
@@ -124,7 +148,9 @@ __device__ __forceinline__ PathCtx path_ctx(const PassArgs& P, const BatchArgs& 
     }
     return PathCtx{BA.pix[p], BA.smp[p], I.key1, I.max_depth, I.rr_start};
   }
-  return PathCtx{p % P.npix, P.sample + p / P.npix, P.key1, P.max_depth, P.rr_start};
+  PathCtx c{0u, 0u, P.key1, P.max_depth, P.rr_start};
+  slot_pixel(P, p, c.pix, c.smp);
+  return c;
 }
 
 // Guided mode: network outputs per queue row of the current bounce and the
@@ -308,8 +334,7 @@ __global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C0, Pat
       BA.pix[i] = pix;
       BA.smp[i] = smp;
     } else {
-      pix = i % P.npix;
-      smp = P.sample + i / P.npix;
+      slot_pixel(P, i, pix, smp);
       key1 = P.key1;
       C = C0;
     }
@@ -334,25 +359,19 @@ __global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C0, Pat
 }
 
 // ------------------------------------------------------------------ extend
-// Nearest hit of every queued path.  Block k runs the contiguous queue range
-// [k*per, (k+1)*per) with trace_range (lanes refill from the block's LDS
-// counter as their rays end); every ray's result is independent of the lane
-// that ran it.
+// Nearest hit of every queued path: the waves take chunks of the queue from
+// the bounce's work counter (trace_range); every ray's result is independent
+// of the lane that ran it.
 template <bool LEG, bool STATS, bool BATCH>
 __global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend(DevTree T, PathBufs B, uint32_t b, BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
-  __shared__ uint32_t blk_next;
   const uint32_t n = B.cnt[b];
   const uint32_t* q = (b & 1) ? B.q1 : B.q0;
-  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
-  const uint32_t lo = min(n, blockIdx.x * per), hi = min(n, lo + per);
-  if (threadIdx.x == 0) blk_next = lo;
-  __syncthreads();
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t boxes = 0, tris = 0;
   bool ovf = false;
   trace_range<false, LEG, STATS, kLdsDepth>(
-      T, hi, &blk_next, stack + threadIdx.x, B.spill + (size_t)gtid * B.spill_depth, B.spill_depth, &ovf,
+      T, n, &B.cnt[kWorkBase + b], stack + threadIdx.x, B.spill + (size_t)gtid * B.spill_depth, B.spill_depth, &ovf,
       [&](uint32_t i, Vec3& o, Vec3& d, int32_t& ins, int32_t& con, TreeRoot& root) {
         const uint32_t p = q[i];
         o = xyz(B.o[p]);
@@ -919,18 +938,13 @@ template <bool BATCH>
 __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
                                                     BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
-  __shared__ uint32_t blk_next;
   const uint32_t n = B.cnt[kShadowBase + b];
-  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
-  const uint32_t lo = min(n, blockIdx.x * per), hi = min(n, lo + per);
-  if (threadIdx.x == 0) blk_next = lo;
-  __syncthreads();
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   uint2* lds = stack + threadIdx.x;
   uint2* spill = B.spill + (size_t)gtid * B.spill_depth;
-  const int lane = threadIdx.x & 63;
   uint32_t rays = 0, srays = 0;
-  bool ovf = false, active = false, drained = false;
+  bool ovf = false, active = false;
+  WaveWork ww = wave_work(n);
   // the lane's query: from o along w to the light point at distance `remaining`
   uint32_t p = 0, max_depth = 0;
   int c = 0, inside = NONE, contact = NONE;
@@ -970,15 +984,10 @@ __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S0, PathBufs B, Pas
   };
   for (;;) {
     const unsigned long long idle = __ballot(!active);
-    if (idle && !drained) {
-      const int leader = __ffsll((long long)idle) - 1;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(&blk_next, (uint32_t)__popcll(idle));
-      base = __shfl(base, leader);
-      if (base + (uint32_t)__popcll(idle) >= hi) drained = true;
-      if (!active) {
-        const uint32_t i = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-        if (i < hi) {
+    if (idle && !ww.drained) {
+      const uint32_t i = wave_take(ww, idle, n, &B.cnt[kShadowWorkBase + b]);
+      {
+        if (i < n) {
           p = B.sq[i];
           if (BATCH) S = S0;  // path_ctx rebinds the scene's root and emitters
           max_depth = path_ctx<BATCH>(P, BA, p, S).max_depth;
@@ -995,7 +1004,7 @@ __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S0, PathBufs B, Pas
         }
       }
     }
-    if (drained && !__any(active)) break;
+    if (ww.drained && !__any(active)) break;
     if (active && trace_trip<false, true, false, kLdsDepth>(S.tree, ts, lds, spill, B.spill_depth, &st, &ovf)) {
       // MCRenderer::testVisibility: visible when the first legal hit is not nearer than the light point
       if (!ts.has || ts.best - remaining >= -1e-5f) {
@@ -1136,10 +1145,11 @@ static __global__ void k_guide_gather(TrainOut S, const uint32_t* total, uint32_
 }
 
 // ------------------------------------------------------------------ film
-// Adds the pass's samples of each pixel in sample order (slot j*npix + pixel
-// holds sample `sample + j`), so the FP64 film is independent of how samples
-// are grouped into passes.
-static __global__ void k_accumulate(PathBufs B, uint32_t npix, uint32_t S) {
+// Adds the pass's samples of each pixel in sample order (slot pass_slot(q, j)
+// holds sample `sample + j` of pixel q), so the FP64 film is independent of
+// how samples are grouped into passes.
+static __global__ void k_accumulate(PathBufs B, PassArgs P) {
+  const uint32_t npix = P.npix, S = P.spass;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < npix; q += stride) {
     double s[3], s2[3];
@@ -1148,7 +1158,7 @@ static __global__ void k_accumulate(PathBufs B, uint32_t npix, uint32_t S) {
       s2[k] = B.sumsq[3 * (size_t)q + k];
     }
     for (uint32_t j = 0; j < S; j++) {
-      const float4 L = B.L[(size_t)j * npix + q];
+      const float4 L = B.L[pass_slot(P, q, j)];
       const double l[3] = {L.x, L.y, L.z};
       for (int k = 0; k < 3; k++) {
         s[k] += l[k];

@@ -238,6 +238,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   PassArgs P;
   P.n_paths = N;
   P.npix = npix;
+  P.spass = 1;
   P.key1 = seed_key(prm->seed);
   P.max_depth = prm->max_depth;
   P.rr_start = prm->rr_start_depth;
@@ -303,6 +304,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     const bool train = mode == RX_MODE_GUIDED && P.sample < train_passes;
     const uint32_t spass = train ? 1u : std::min(S_max, prm->spp - done);
     P.n_paths = npix * spass;
+    P.spass = spass;
     done += spass;
     GA.tag = train ? (float)(pass + 1) : 0.0f;
     hipLaunchKernelGGL((k_generate<false>), dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, BA0);
@@ -339,7 +341,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
         if (mode) sdw.push_back({e2, e3});
       }
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(kRBlock), 0, s, B, npix, spass);
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(kRBlock), 0, s, B, P);
     RXCK(hipGetLastError(), "launch pass");
     if (GA.tag > 0.0f) {  // online training on this pass's radiance samples
       GuideState& G = *c->guide;

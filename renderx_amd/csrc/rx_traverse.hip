@@ -2,8 +2,8 @@
 // (Shader/Intersection.frag:25-84, one fragment per ray) with the CPU
 // semantics of KDTree::intersect (src/KDTree.cpp:295-339).
 //
-// Each block runs a contiguous range of the batch, one ray per lane, and
-// lanes take the block's next ray as theirs ends (trace_range); the
+// Each wave takes chunks of the batch from a shared counter, one ray per
+// lane, and lanes take the chunk's next ray as theirs ends (trace_range); the
 // traversal stack is a per-lane LDS column (conflict-free: lane-contiguous)
 // with a per-thread global spill for deep trees.
 #include "rx_kernels.h"
@@ -13,15 +13,10 @@ namespace rx {
 template <bool EXH, bool LEG, bool STATS>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(DevTree T, IsectArgs a) {
   __shared__ uint2 stack[kLdsDepth * kTraceBlock];
-  __shared__ uint32_t blk_next;
-  const uint32_t per = (a.n + gridDim.x - 1) / gridDim.x;
-  const uint32_t lo = min(a.n, blockIdx.x * per), hi = min(a.n, lo + per);
-  if (threadIdx.x == 0) blk_next = lo;
-  __syncthreads();
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   bool ovf = false;
   trace_range<EXH, LEG, STATS, kLdsDepth>(
-      T, hi, &blk_next, stack + threadIdx.x, a.spill + (size_t)gtid * a.spill_depth, a.spill_depth, &ovf,
+      T, a.n, a.work, stack + threadIdx.x, a.spill + (size_t)gtid * a.spill_depth, a.spill_depth, &ovf,
       [&](uint32_t i, Vec3& o, Vec3& d, int32_t& ins, int32_t& con, TreeRoot&) {
         o = v3(a.org[3 * (size_t)i], a.org[3 * (size_t)i + 1], a.org[3 * (size_t)i + 2]);
         d = v3(a.dir[3 * (size_t)i], a.dir[3 * (size_t)i + 1], a.dir[3 * (size_t)i + 2]);

@@ -45,6 +45,11 @@ def test_guided_mean_matches_plain(rx, gpu_ctx, scene):
     assert frac >= 0.995 and abs(zmean) < 0.1
 
 
+# samples per pixel of one render call in the equal-time loops: large enough
+# that a call's time is rendering rather than fixed per-call cost
+CHUNK = 256
+
+
 def _equal_time(pt, mode, budget_s, chunk, seed):
     s = None
     q = None
@@ -69,8 +74,9 @@ def _reference64(rx, ctx):
     if "ref" not in _ref64:
         t = time.perf_counter()
         pt = rx.PathTracerHIP(_media64(), ctx=ctx)
-        _ref64["ref"] = pt.renderPixels(spp=16384, mode=MODE_NEE, seed=99)["sum"] / 16384
-        print(f"  reference (NEE, 16384 spp) in {time.perf_counter() - t:.1f} s, mean {_ref64['ref'].mean():.5f}")
+        n = 1 << 18
+        _ref64["ref"] = pt.renderPixels(spp=n, mode=MODE_NEE, seed=99)["sum"] / n
+        print(f"  reference (NEE, {n} spp) in {time.perf_counter() - t:.1f} s, mean {_ref64['ref'].mean():.5f}")
     return _ref64["ref"]
 
 
@@ -78,11 +84,11 @@ def test_equal_time_relmse_guided_vs_plain(rx, gpu_ctx):
     print("[guided] equal-time relMSE", flush=True)
     sc = _media64()
     ref = _reference64(rx, gpu_ctx)
-    budget = 3.0
+    budget = 2.0
     res = {}
     for name, mode in [("plain", MODE_PT), ("nee", MODE_NEE), ("guided", MODE_GUIDED)]:
         pt = _guided(rx, gpu_ctx, sc) if mode == MODE_GUIDED else rx.PathTracerHIP(sc, ctx=gpu_ctx)
-        img, n, dt = _equal_time(pt, mode, budget, 16, seed=5)
+        img, n, dt = _equal_time(pt, mode, budget, CHUNK, seed=5)
         res[name] = rel_mse(img, ref)
         print(f"  {name:7s}: {n:6d} spp in {dt:.2f} s, relMSE {res[name]:.5f}")
     print(f"  guided vs plain: {res['plain'] / res['guided']:.2f}x, guided vs NEE-only: {res['nee'] / res['guided']:.2f}x")
@@ -99,7 +105,7 @@ def test_trained_network_beats_untrained(rx, gpu_ctx):
     res = {}
     for name, passes in [("untrained", 0), ("trained", 16)]:
         pt = _guided(rx, gpu_ctx, sc, passes=passes)
-        img, n, dt = _equal_time(pt, MODE_GUIDED, 3.0, 16, seed=5)
+        img, n, dt = _equal_time(pt, MODE_GUIDED, 2.0, CHUNK, seed=5)
         res[name] = rel_mse(img, ref)
         print(f"  {name:9s}: {n:6d} spp in {dt:.2f} s, relMSE {res[name]:.5f}")
     print(f"  network gain {res['untrained'] / res['trained']:.2f}x")
