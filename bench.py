@@ -8,7 +8,8 @@ before the timed region; the tree is built once outside it.
 
 Prints ONE JSON line with the metric, the extend (traversal) kernel's roofline
 figures, the equal-time relMSE of the plain and the guided renderer on the
-media + glass scene, the throughput of the 64-scene batch, and the CPU baseline (the oracle's restated PathTracer on
+media + glass scene, the throughput of the 64-scene batch and of the
+bidirectional tracer, and the CPU baseline (the oracle's restated PathTracer on
 this host's cores over a bounded sample of the same scene).
 """
 from __future__ import annotations
@@ -143,6 +144,43 @@ def batch64(steps=2):
             "host_tree_build_s": build_s}
 
 
+def bpt_cornell(res=256, spp=64, cpu=True):
+    """The bidirectional tracer (RX_MODE_BPT) on the Cornell box with a glass
+    and a Phong sphere (res^2 px, max depth 8, Russian roulette at every
+    bounce): samples/s and traversal queries/s on the GPU, and the oracle's
+    restated BidirectionalPathTracer on this host's cores over a bounded
+    sample (64x64 px) of the same scene."""
+    from renderx_amd import api, scenes
+
+    sc = scenes.glass_scene(res, res, max_depth=8, glass_freq=16, phong_freq=16)
+    pt = api.BidirectionalPathTracerHIP(sc, ctx=api.Context(0))
+    pt.renderPixels(spp=4, seed=1, want_sums=False)  # warm-up
+    r = pt.renderPixels(spp=spp, seed=2, want_sums=False)
+    st = r["stats"]
+    out = {"scene": f"Cornell box + glass + Phong sphere {sc.n_tris} tris, {res}x{res} px, {spp} spp, max depth 8",
+           "ms": st["ms_total"], "samples_per_s": st["samples"] / st["ms_total"] * 1e3,
+           "rays_per_s": st["rays"] / st["ms_total"] * 1e3, "shadow_rays_per_sample": st["shadow_rays"] / st["samples"]}
+    log(f"[bench] bpt: {spp} spp in {st['ms_total']:.0f} ms, {out['samples_per_s'] / 1e6:.1f} M samples/s")
+    if cpu:
+        from oracle import oracle
+
+        small = scenes.glass_scene(64, 64, max_depth=8, glass_freq=16, phong_freq=16)
+        so = oracle.SceneOracle(small)
+        n = 4
+        t = time.perf_counter()
+        so.render(n, mode=3, seed=3)
+        dt = time.perf_counter() - t
+        n = max(n, int(n * 10.0 / max(dt, 1e-3)))  # about 10 s of CPU work
+        t = time.perf_counter()
+        so.render(n, mode=3, seed=4)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": 64 * 64 * n / dt, "unit": "samples/s",
+                               "cores": int(oracle.lib().orc_max_threads()), "kind": "port",
+                               "sample": f"oracle BidirectionalPathTracer, 64x64 px, {n} spp ({dt:.1f} s)"}
+        log(f"[bench] bpt cpu baseline: {out['cpu_baseline']['value'] / 1e3:.1f} k samples/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +191,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-relmse", action="store_true", help="skip the equal-time relMSE study")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch")
+    ap.add_argument("--no-bpt", action="store_true", help="skip the bidirectional tracer")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -234,6 +273,8 @@ def main():
         out["equal_time_relmse"] = equal_time_relmse()
     if not args.no_batch:
         out["batch64"] = batch64()
+    if not args.no_bpt:
+        out["bpt"] = bpt_cornell(cpu=not args.no_cpu)
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(scene)

@@ -70,7 +70,12 @@ typedef struct rx_camera_desc {
 enum rx_render_mode {
   RX_MODE_PT = 0,      /* PathTracer: BSDF sampling only, emitter-terminated paths (src/PathTracer.cpp:17-89) */
   RX_MODE_NEE = 1,     /* next-event estimation to area lights + MIS, unbiased w.r.t. RX_MODE_PT */
-  RX_MODE_GUIDED = 2   /* RX_MODE_NEE + learned vMF-mixture proposal combined by one-sample MIS */
+  RX_MODE_GUIDED = 2,  /* RX_MODE_NEE + learned vMF-mixture proposal combined by one-sample MIS */
+  RX_MODE_BPT = 3      /* BidirectionalPathTracer (src/BidirectionalPathTracer.cpp:6-398): per pixel and
+                          sample one eye and one light subpath, every connection weighted by the reference's
+                          balance heuristic; light-tracing contributions land on the pixel
+                          Camera::transToPixel names.  Surface scenes (no RX_OBJ_VPM), max_depth <= 30,
+                          at most 2^22 pixels */
 };
 
 typedef struct rx_render_params {
@@ -85,7 +90,11 @@ typedef struct rx_render_params {
 
 enum rx_render_flags {
   RX_RENDER_TIMING = 1,     /* time each kernel with HIP events (ms_extend/ms_shade/ms_shadow) */
-  RX_RENDER_COUNT_WORK = 2  /* count box and triangle tests of the extend kernels */
+  RX_RENDER_COUNT_WORK = 2, /* count box and triangle tests of the extend kernels */
+  RX_RENDER_BPT_REGISTERED = 4 /* RX_MODE_BPT: light-tracing contributions land on the pixel whose camera rays
+                                  cover them (floor of the raster position) instead of transToPixel's
+                                  (int)(x - 0.5), which shifts them by half a pixel and doubles the first
+                                  row's and column's footprint; the image then has RX_MODE_PT's expectation */
 };
 
 /* Launch shape of rx_render; no counterpart in the reference.  Every setting

@@ -152,6 +152,7 @@ struct Ctx {
   Rng& g;
   unsigned maxDepth;
   unsigned rrStart;
+  bool light = false;  // scatter's fixIsLight: a light subpath of the BidirectionalPathTracer
 };
 
 static inline bool is_emissive(const SceneO& sc, int o) { return o >= 0 && sc.objs[o].type == OBJ_EMISSIVE; }
@@ -355,7 +356,7 @@ static PRay refractive_scatter(Ctx& c, int object, const PRay& in, bool russian)
     out.directionProb = p;
     out.insideObject = in.insideObject;
   } else {
-    out.color = out.color * ((current_n * current_n) / (next_n * next_n));  // eye path: fixIsLight=false
+    if (!c.light) out.color = out.color * ((current_n * current_n) / (next_n * next_n));  // eye paths only
     out.color = out.color * ((1 - er) / cosine_term(c.sc, out));
     out.directionProb = 1 - p;
     out.insideObject = outSide;
@@ -695,6 +696,7 @@ static V3 obj_bsdf(const SceneO& sc, int o, const PRay& in, const PRay& out) {
   return V3(0, 0, 0);  // refractive: Material::getBSDF
 }
 static V3 get_bsdf(const SceneO& sc, const PRay& in, const PRay& out) {  // Ray::getBSDF
+  if (out.contactObject == CAMERA) return V3(1, 1, 1);  // Camera::getBSDF (include/Camera.h:31)
   if (out.contactObject >= 0) return obj_bsdf(sc, out.contactObject, in, out);
   if (out.insideObject >= 0) return obj_bsdf(sc, out.insideObject, in, out);
   return V3(0, 0, 0);
@@ -727,6 +729,7 @@ static float obj_dir_pdf(const SceneO& sc, int o, const PRay& in, const PRay& ou
   return 0;
 }
 static float dir_pdf(const SceneO& sc, const PRay& in, const PRay& out) {
+  if (out.contactObject == CAMERA) return 1;  // Camera::getDirectionSampleProbDensity (src/Camera.cpp:99-108)
   if (out.contactObject >= 0) return obj_dir_pdf(sc, out.contactObject, in, out);
   if (out.insideObject >= 0) return obj_dir_pdf(sc, out.insideObject, in, out);
   return 0;
@@ -842,7 +845,258 @@ static V3 pttest_sample(Ctx& c, unsigned pixel, std::vector<PRay>& path) {
   return color;
 }
 
+// ---- BidirectionalPathTracer, CPU branch (src/BidirectionalPathTracer.cpp:6-398)
+typedef std::vector<PRay> Path;
+
+// A ray's density of its own direction, Ray::getDirectionSampleProbDensity(ray)
+// called with itself: the emitted-ray branch of SceneObject's
+// (src/SceneObject.cpp:234-247, cosine about the contact normal); Camera
+// overrides it with 1 (src/Camera.cpp:99-108).
+static float self_dir_pdf(const SceneO& sc, const PRay& r) {
+  if (r.contactObject == CAMERA) return 1.f;
+  if (r.contactObject < 0) return 0.f;
+  return cos_pdf(frame_from(contact_normal(sc, r)), r.direction, 1.f);
+}
+// Ray::getOriginSampleProbDensity (src/Ray.cpp:42-50): 1 unless the ray's
+// inside object is a medium, and media are rejected by this renderer.
+static float origin_pdf(const SceneO&, const PRay&, const PRay&) { return 1.f; }
+
+// BidirectionalPathTracer::mustUsePT (:124-131)
+static bool must_use_pt(const Path& path) {
+  for (size_t i = 0; i + 1 < path.size(); i++)
+    if (path[i].directionSampleType == path[i + 1].directionSampleType && path[i].directionSampleType == RANDOM)
+      return false;
+  return true;
+}
+
+// BidirectionalPathTracer::link (:183-195): the vertex i ray, turned toward
+// vertex j unless it already points that way along the connected path.
+static PRay bpt_link(const Path& path, int ci, int i, int j) {
+  const int n = (int)path.size();
+  const PRay& src = path[std::min(std::max(i, 0), n - 1)];
+  const PRay& dst = path[std::min(std::max(j, 0), n - 1)];
+  PRay r = src;
+  if ((j > i && i <= ci) || (j < i && i > ci)) return r;
+  r.direction = normalize(dst.origin - src.origin);
+  r.insideObject = dst.insideObject;
+  return r;
+}
+
+// MCRenderer::connectRays (src/MCRenderer.cpp:50-106), merged = false
+static bool connect_rays(const SceneO& sc, Path& path, int ci) {
+  PRay& lightRay = path[ci];
+  PRay& eyeRay = path[ci + 1];
+  if (lightRay.directionSampleType == DEFINITE || eyeRay.directionSampleType == DEFINITE) return false;
+  const int n = (int)path.size();
+  const PRay& prevLightRay = path[std::max(ci - 1, 0)];
+  const PRay& prevEyeRay = path[std::min(ci + 2, n - 1)];
+  lightRay.direction = normalize(eyeRay.origin - lightRay.origin);
+  eyeRay.direction = -lightRay.direction;
+  lightRay.directionProb = 1;
+  lightRay.color = get_bsdf(sc, prevLightRay, lightRay);
+  lightRay.directionSampleType = RANDOM;
+  if (lightRay.contactObject != NONE && ci > 0) {
+    if (dot(contact_normal(sc, lightRay), lightRay.direction) <= 0) lightRay.insideObject = lightRay.contactObject;
+    else if (dot(contact_normal(sc, lightRay), prevLightRay.direction) <= 0) lightRay.insideObject = prevLightRay.insideObject;
+    else lightRay.insideObject = find_inside(sc, lightRay.origin, lightRay.direction, lightRay.contactObject);
+  }
+  eyeRay.directionProb = 1;
+  eyeRay.color = get_bsdf(sc, prevEyeRay, eyeRay);
+  eyeRay.directionSampleType = RANDOM;
+  if (eyeRay.contactObject != NONE && ci < n - 2) {
+    if (dot(contact_normal(sc, eyeRay), eyeRay.direction) <= 0) eyeRay.insideObject = eyeRay.contactObject;
+    else if (dot(contact_normal(sc, eyeRay), prevEyeRay.direction) <= 0) eyeRay.insideObject = prevEyeRay.insideObject;
+    else eyeRay.insideObject = find_inside(sc, eyeRay.origin, eyeRay.direction, eyeRay.contactObject);
+  }
+  return true;
+}
+
+// BidirectionalPathTracer::connectColorProb (:133-181): (throughput, density)
+static void connect_color_prob(const SceneO& sc, const Path& path, int ci, V3& color, float& prob) {
+  color = V3(1, 1, 1);
+  prob = 1;
+  const int n = (int)path.size();
+  for (int i = 0; i < n; i++) {
+    float dist = 0;
+    if (i <= ci) {
+      dist = std::max(length(path[i + 1].origin - path[i].origin), EPS);
+      color = color * radiance_decay(sc, path[i], dist);
+    } else if (i > ci + 1) {
+      dist = std::max(length(path[i - 1].origin - path[i].origin), EPS);
+      color = color * radiance_decay(sc, path[i], dist);
+    }
+    if (i == ci && i < n - 1) color = color * (cosine_term(sc, path[i]) * cosine_term(sc, path[i + 1]) / (dist * dist));
+    if (i != ci && i != ci + 1) color = color * cosine_term(sc, path[i]);
+    prob *= path[i].directionProb * path[i].originProb;
+    color = color * path[i].color;
+  }
+}
+
+// BidirectionalPathTracer::connectWeight (:197-263) with expTerm = 1: the
+// balance heuristic over the strategies that could have made the path
+static float connect_weight(const SceneO& sc, const Path& path, int ci, bool usePT) {
+  const int n = (int)path.size();
+  std::vector<double> pf(n), pb(n), dl(n);
+  pf[0] = path[0].originProb;
+  for (int i = 0; i < n - 1; i++) {
+    pf[i + 1] = pf[i] * cosine_term(sc, bpt_link(path, ci, i + 1, i));
+    dl[i] = std::max(length(path[i + 1].origin - path[i].origin), EPS);
+    pf[i + 1] /= dl[i] * dl[i];
+    if (path[i].directionSampleType == RANDOM) {
+      if (i > 0) pf[i + 1] *= dir_pdf(sc, bpt_link(path, ci, i - 1, i), bpt_link(path, ci, i, i + 1));
+      else pf[i + 1] *= self_dir_pdf(sc, bpt_link(path, ci, i, i + 1));
+      pf[i + 1] *= origin_pdf(sc, bpt_link(path, ci, i, i + 1), bpt_link(path, ci, i + 1, i + 2));
+    }
+  }
+  pb[n - 1] = path[n - 1].originProb;
+  for (int i = n - 1; i > 0; i--) {
+    pb[i - 1] = pb[i] * cosine_term(sc, bpt_link(path, ci, i - 1, i));
+    pb[i - 1] /= dl[i - 1] * dl[i - 1];
+    if (path[i].directionSampleType == RANDOM) {
+      if (i < n - 1) pb[i - 1] *= dir_pdf(sc, bpt_link(path, ci, i + 1, i), bpt_link(path, ci, i, i - 1));
+      else pb[i - 1] *= self_dir_pdf(sc, bpt_link(path, ci, i, i - 1));
+      pb[i - 1] *= origin_pdf(sc, bpt_link(path, ci, i, i - 1), bpt_link(path, ci, i - 1, i - 2));
+    }
+  }
+  double sum = 0;
+  for (int i = 0; i < n - 1; i++)
+    if (path[i].directionSampleType == RANDOM && path[i + 1].directionSampleType == RANDOM) sum += pf[i] * pb[i + 1];
+  if (usePT || must_use_pt(path)) sum += pb[0];
+  double self = ci == -1 ? pb[0] : pf[ci] * pb[ci + 1];
+  if (!(self / sum >= 0)) return 0;
+  return (float)(self / sum);
+}
+
+// Camera::transToPixel (src/Camera.cpp:136-152)
+static void trans_to_pixel(const SceneO& sc, V3 point, float& px, float& py) {
+  V3 v = point - sc.cam.position;
+  float frontDist = dot(v, sc.cam_front);
+  float ratio = sc.cam.sightDist / frontDist;
+  px = dot(v, sc.cam_right) * ratio;
+  py = dot(v, sc.cam_up) * ratio;
+  px += sc.cam.width / 2.f - 0.5f;
+  py = sc.cam.height / 2.f - py - 0.5f;
+}
+
+// BidirectionalPathTracer::colorByConnectingPaths (:265-396): every (light
+// prefix, eye prefix) pair of one eye and one light subpath; contributions
+// with an eye prefix of one vertex land on the pixel the light vertex projects
+// to, the others on the eye path's own pixel.  img is the iteration's image.
+// With `registered`, a light-tracing contribution lands on the pixel whose
+// camera rays cover its raster position (floor, not the reference's
+// (int)(x - 0.5) of transToPixel's pixel-centre coordinates); the estimator
+// then has the PathTracer's expectation at every pixel, which is what the
+// tests use it for.
+static void connect_paths(Ctx& c, unsigned pixel, const Path& eyePath, const Path& lightPath, double* img,
+                          bool usePT, bool registered) {
+  const SceneO& sc = c.sc;
+  const int maxE = (int)eyePath.size(), maxL = (int)lightPath.size();
+  const int npix = sc.cam.width * sc.cam.height;
+  for (int W = 2; W <= maxE + maxL; W++) {
+    const int lplStart = W - maxE > 0 ? W - maxE : 0;
+    const int lplEnd = W - 1 < maxL ? W - 1 : maxL;
+    for (int lp = lplStart; lp <= lplEnd; lp++) {
+      const int ep = W - lp;
+      Path path(W);
+      for (int li = 0; li < lp; li++) path[li] = lightPath[li];
+      for (int ei = 0; ei < ep; ei++) path[W - 1 - ei] = eyePath[ei];
+      const int lci = lp - 1, eci = lp;
+      if (lp == 0 && !is_emissive(sc, path.front().contactObject)) continue;
+      if (lp == 0 && !(usePT || must_use_pt(path))) continue;
+      if (ep > 0 && lp > 0) {
+        if (lp > 1 && is_emissive(sc, path[lci].contactObject)) continue;
+        if (is_emissive(sc, path[eci].contactObject)) continue;
+        if (!connect_rays(sc, path, lci)) continue;
+        if (!test_visibility(sc, path[eci], path[lci])) continue;
+      }
+      V3 color;
+      float prob;
+      connect_color_prob(sc, path, lci, color, prob);
+      if (!(prob > 0)) continue;
+      if (!(length(color) > 0)) continue;
+      float weight = connect_weight(sc, path, lci, usePT);
+      color = color / prob * weight;
+      if (ep == 1) {
+        V3 dir = normalize(path[lci].origin - path[eci].origin);
+        float cosAtCamera = std::fabs(dot(sc.cam_front, dir));
+        float imageToSolidAngleFactor = (float)(sc.cam.sightDist * sc.cam.sightDist) / (cosAtCamera * cosAtCamera * cosAtCamera);
+        float imageToSurfaceFactor = imageToSolidAngleFactor / cosAtCamera;
+        color = color * (imageToSurfaceFactor / (float)npix);
+        color = color * cosAtCamera;
+        float fx, fy;
+        trans_to_pixel(sc, path[W - 2].origin, fx, fy);
+        int x = (int)fx, y = (int)fy;
+        if (registered) {
+          x = (int)std::floor(fx + 0.5f);
+          y = (int)std::floor(fy + 0.5f);
+        }
+        if (x >= 0 && x < sc.cam.width && y >= 0 && y < sc.cam.height) {
+          double* d = img + 3 * ((size_t)y * sc.cam.width + x);
+          for (int k = 0; k < 3; k++) {
+#pragma omp atomic
+            d[k] += color[k];
+          }
+        }
+      } else {
+        double* d = img + 3 * (size_t)pixel;
+        for (int k = 0; k < 3; k++) {
+#pragma omp atomic
+          d[k] += color[k];
+        }
+      }
+    }
+  }
+}
+
+// One BidirectionalPathTracer iteration per sample: per pixel one eye and
+// one light subpath (src/BidirectionalPathTracer.cpp:36-57), the iteration
+// image divided by the vignetting factor (:100) and added to the FP64 sums.
+static void render_bpt(const SceneO& sc, const RenderParamsO& p, double* sum, double* sumsq, uint64_t* rays) {
+  const unsigned npix = (unsigned)(sc.cam.width * sc.cam.height);
+  std::vector<double> img(3 * (size_t)npix);
+  uint64_t total_rays = 0;
+  for (unsigned smp = 0; smp < p.spp; smp++) {
+    std::fill(img.begin(), img.end(), 0.0);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total_rays)
+    for (int px = 0; px < (int)npix; px++) {
+      g_rays = 0;
+      Rng g(p.seed, (uint64_t)px, (uint64_t)(smp + p.sample_offset));
+      Ctx c{sc, g, p.max_depth, p.rr_start_depth};
+      Path eyePath, lightPath;
+      PRay lightRay = light_sample(c);
+      if (length(lightRay.direction) > 0.5f) {  // SceneObject::emit's own first hit, with the condition (:212-223)
+        int obj = NONE;
+        unsigned lt = 0;
+        float dist = scene_intersect(sc, lightRay, true, obj, lt);
+        if (dist > 0) {
+          lightRay.intersectDist = dist;
+          lightRay.intersectObject = obj;
+          lightRay.intersectTri = lt;
+        }
+      }
+      PRay cameraRay = camera_ray(c, px);
+      sample_path(c, cameraRay, eyePath);
+      c.light = true;
+      sample_path(c, lightRay, lightPath);
+      c.light = false;
+      connect_paths(c, (unsigned)px, eyePath, lightPath, img.data(), false, (p.flags & 1u) != 0);
+      total_rays += g_rays;
+    }
+    for (unsigned px = 0; px < npix; px++)
+      for (int k = 0; k < 3; k++) {
+        double v = (float)(img[3 * px + k]) / sc.vignette[px];
+        sum[3 * px + k] += v;
+        if (sumsq) sumsq[3 * px + k] += v * v;
+      }
+  }
+  if (rays) *rays += total_rays;
+}
+
 void render(const SceneO& sc, const RenderParamsO& p, double* sum, double* sumsq, uint64_t* rays) {
+  if (p.mode == 3) {
+    render_bpt(sc, p, sum, sumsq, rays);
+    return;
+  }
   const unsigned npix = (unsigned)(sc.cam.width * sc.cam.height);
   uint64_t total_rays = 0;
 #pragma omp parallel for schedule(static) reduction(+ : total_rays)

@@ -27,7 +27,7 @@ struct CamDescC {
   int32_t width, height;
 };
 struct RenderParamsO {
-  uint32_t spp, max_depth, rr_start_depth, mode;  // mode 0 = PathTracer, 1 = PathTracerTest
+  uint32_t spp, max_depth, rr_start_depth, mode;  // 0 PathTracer, 1 PathTracerTest, 3 BidirectionalPathTracer
   uint64_t seed;
   uint32_t sample_offset, flags;
 };

@@ -29,7 +29,7 @@ RENDER_TIMING, RENDER_COUNT_WORK = 1, 2
 RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use_graph", "<u4"), ("reserved", "<u4")])
 GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("batch_size", "<u4"), ("steps_per_pass", "<u4"), ("learning_rate", "<f4"),
                          ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4")])
-MODE_PT, MODE_NEE, MODE_GUIDED = 0, 1, 2
+MODE_PT, MODE_NEE, MODE_GUIDED, MODE_BPT = 0, 1, 2, 3
 
 
 class KDTree:
@@ -189,6 +189,16 @@ class PathTracerHIP:
         self.ctx.check(lib().rx_render(self.ctx.h, ptr(cam), ptr(p), ptr(mean), ptr(s), ptr(q), ptr(st)),
                        "rx_render")
         return {"mean": mean, "sum": s, "sumsq": q, "stats": {k: st[k][0].item() for k in RENDER_STATS.names}}
+
+
+class BidirectionalPathTracerHIP(PathTracerHIP):
+    """MCRenderer plugin "BPT" (src/BidirectionalPathTracer.cpp): renderPixels
+    renders with RX_MODE_BPT unless another mode is asked for."""
+
+    def renderPixels(self, camera=None, spp=None, max_depth=None, rr_start_depth=None, mode=MODE_BPT, seed=1,
+                     sample_offset=0, want_sums=True, flags=0):
+        return super().renderPixels(camera, spp, max_depth, rr_start_depth, mode, seed, sample_offset, want_sums,
+                                    flags)
 
 
 class BatchRenderer:

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librxhip.so")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["rx_capi.hip", "rx_traverse.hip", "rx_render.hip", "rx_batch.hip", "rx_guide.hip"]
+HIP_SOURCES = ["rx_capi.hip", "rx_traverse.hip", "rx_render.hip", "rx_bpt.hip", "rx_batch.hip", "rx_guide.hip"]
 CXX_SOURCES = ["rx_tree.cpp", "rx_film.cpp", "rx_scene_io.cpp"]
 CMP_OUT = os.path.join(HERE, "rx_cmp")  # the cmp tool (host only)
 

@@ -314,17 +314,17 @@ int rx_batch_render(rx_batch* b, rx_render_stats* stats) {
   hipLaunchKernelGGL(k_pass_set, dim3(1), dim3(64), 0, s, BA.pass_id, 0u);
   RXCK(hipGetLastError(), "launch fill");
   auto launch_pass = [&]() {
-    hipLaunchKernelGGL((k_generate<true>), dim3(blocks), dim3(kRBlock), 0, s, S, C0, B, P, BA);
+    hipLaunchKernelGGL((k_generate<true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, C0, B, P, BA);
     for (uint32_t bb = 0; bb <= Dmax; bb++) {
       if (bb == 0)
         hipLaunchKernelGGL((k_extend<false, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, bb, BA);
       else
         hipLaunchKernelGGL((k_extend<true, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, bb, BA);
       if (b->mode == RX_MODE_NEE) {
-        hipLaunchKernelGGL((k_shade<1, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
-        hipLaunchKernelGGL((k_shadow<true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
+        hipLaunchKernelGGL((k_shade<1, true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
+        hipLaunchKernelGGL((k_shadow<true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
       } else {
-        hipLaunchKernelGGL((k_shade<0, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
+        hipLaunchKernelGGL((k_shade<0, true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
       }
     }
     hipLaunchKernelGGL(k_accumulate_batch, dim3(blocks), dim3(kRBlock), 0, s, B, BA);

@@ -205,8 +205,13 @@ __device__ __forceinline__ float tri_dist(Vec3 v0, Vec3 v1, Vec3 v2, Vec3 o, Vec
   return dist;  // may be NaN for degenerate triangles: the caller's dist >= 0 rejects it
 }
 
+// A contact id that stands for "no condition": the query takes the nearest hit
+// as Scene::intersect without a Condition does (camera rays in a traversal
+// that applies the condition to the other rays of its queue).
+constexpr int32_t kNoCondition = -3;
 // NoSelfIntersectionCondition::legal (src/NoSelfIntersectionCondition.cpp:12-27)
 __device__ __forceinline__ bool legal(Vec3 d, Vec3 n, int32_t obj, int32_t inside, int32_t contact) {
+  if (contact == kNoCondition) return true;
   float dn = dot(d, n);
   bool in = dn <= 0, out = dn >= 0;
   if (inside == contact) return !(in && contact == obj);
@@ -392,26 +397,32 @@ __device__ __forceinline__ Hit trace(const DevTree& T, Vec3 o, Vec3 d, int32_t i
 }
 
 #ifndef RX_CHUNK
-#define RX_CHUNK 256
+#define RX_CHUNK 64
 #endif
 constexpr uint32_t kChunk = RX_CHUNK;  // most queue entries a wave takes from the shared counter at a time
 
-// A wave's share of a queue [0, n): the waves take chunks of neighbouring
-// entries from a shared counter (zero at launch), and a wave hands its
-// chunk's entries to its idle lanes.  Neighbouring entries stay on one wave,
-// and the counter keeps every wave busy until the queue is drained, however
-// unevenly the costs fall over the queue.  Chunks hold about a quarter of an
-// even share of the queue per wave (at most kChunk), so that short queues
-// still spread over the whole grid.
+// A wave's share of a queue [0, n).  Every wave starts on its own chunk of
+// neighbouring entries (wave w: [w*ch, (w+1)*ch)), then takes further chunks
+// from a shared counter (zero at launch) over the rest of the queue, and
+// hands a chunk's entries to its idle lanes.  Neighbouring entries stay on one
+// wave, and the counter keeps every wave busy until the queue is drained,
+// however unevenly the costs fall over the queue.  A chunk is an even share
+// of the queue per wave, capped at kChunk: a short queue is spread thinly over
+// the whole grid, starts all of its rays at once and needs no atomics.
 struct WaveWork {
   uint32_t next = 0, end = 0;  // the wave's chunk [next, end), wave-uniform
   uint32_t ch = 1;             // chunk size
-  bool drained = false;        // the counter has passed n
+  uint32_t dyn = 0;            // start of the counter's part of the queue
+  bool drained = false;        // nothing left for this wave
 };
 __device__ __forceinline__ WaveWork wave_work(uint32_t n) {
   WaveWork w;
-  const uint32_t quarters = 4u * gridDim.x * (blockDim.x >> 6);
-  w.ch = max(1u, min(kChunk, (n + quarters - 1) / quarters));
+  const uint32_t per_block = blockDim.x >> 6, waves = gridDim.x * per_block;
+  const uint32_t wid = blockIdx.x * per_block + (threadIdx.x >> 6);
+  w.ch = max(1u, min(kChunk, (n + waves - 1) / waves));
+  w.next = min(n, wid * w.ch);
+  w.end = min(n, w.next + w.ch);
+  w.dyn = waves * w.ch;
   return w;
 }
 // Called by every lane of a wave with the wave's ballot of idle lanes
@@ -424,9 +435,11 @@ __device__ __forceinline__ uint32_t wave_take(WaveWork& w, unsigned long long id
   const uint32_t avail = w.end - w.next;
   uint32_t i = w.next + rank;
   if (need > avail) {  // the chunk runs short: the next idle lanes take a new one
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(work, w.ch);
-    base = __shfl(base, 0);
+    uint32_t base = n;
+    if (w.dyn < n) {
+      if (lane == 0) base = w.dyn + atomicAdd(work, w.ch);
+      base = __shfl(base, 0);
+    }
     if (rank >= avail) i = (rank - avail < w.ch) ? base + (rank - avail) : n;
     if (base >= n) {
       w.drained = true;

@@ -20,6 +20,7 @@ namespace rx {
 
 enum { OBJ_DIFFUSE = 0, OBJ_GLOSSY = 1, OBJ_PHONG = 2, OBJ_REFRACTIVE = 3, OBJ_EMISSIVE = 4, OBJ_VPM = 5 };
 constexpr int NONE = -1, CAMERA = -2;
+static_assert(kNoCondition != NONE && kNoCondition != CAMERA, "distinct contact sentinels");
 constexpr int kRBlock = kTraceBlock;  // render kernels share the traversal block shape
 #ifndef RX_EXTEND_MIN_WAVES
 #define RX_EXTEND_MIN_WAVES 7
@@ -30,7 +31,7 @@ constexpr int kMaxBounce = 64;
 constexpr int kShadowBase = 96;       // cnt[kShadowBase + b]: shadow queue length of bounce b
 constexpr int kWorkBase = 192;       // cnt[kWorkBase + b]: extend work counter of bounce b
 constexpr int kShadowWorkBase = 256; // cnt[kShadowWorkBase + b]: shadow work counter of bounce b
-constexpr int kCntWords = 320;
+constexpr int kCntWords = 324;
 constexpr uint32_t kDefaultPathSlots = 1u << 25;  // rx_render_tuning.path_slots default (~5.7 GB of path state)
 constexpr uint32_t kGuidedPathSlots = 1u << 22;   // guided mode: the network's per-row buffers scale with the slots
 constexpr int kMaxCross = 4;          // index-matched boundary crossings per shadow ray
@@ -301,7 +302,7 @@ __device__ __forceinline__ float4 f4(Vec3 v, float w) { return make_float4(v.x, 
 // Camera::generateRay / generateRayBase (src/Camera.cpp:38-47,4-36); the
 // camera cosine and the vignetting division of src/PathTracer.cpp:55,67
 // (Camera::eliminateVignetting, src/Camera.cpp:78-96) start the throughput.
-template <bool BATCH>
+template <bool BATCH, bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C0, PathBufs B, PassArgs P, BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -347,7 +348,7 @@ __global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C0, Pat
     Vec3 dc = normalize(front * C.sight + right * ((float)x + 0.5f - (float)C.W * 0.5f) +
                         up * ((float)C.H * 0.5f - 0.5f - (float)y));
     float beta = fabsf(dot(front, d)) / fabsf(dot(dc, front));
-    int inside = (!BATCH && S.n_vol) ? find_inside(S, pos, d, NONE, stack + threadIdx.x, B, rays) : NONE;
+    int inside = (VOL && !BATCH && S.n_vol) ? find_inside(S, pos, d, NONE, stack + threadIdx.x, B, rays) : NONE;
     B.o[i] = f4(pos, 0.0f);
     B.d[i] = f4(d, 0.0f);
     B.s[i] = make_int4(inside, CAMERA, -1, 0);
@@ -540,9 +541,13 @@ __device__ __forceinline__ void surface_bsdf(const DevObject& O, Vec3 n, Vec3 d,
 }
 
 // One scatter of a surface object (the Material::scatter implementations).
+// light: the scatter of a light subpath (fixIsLight), which leaves out the
+// refracted radiance's (n_i/n_t)^2 scaling.  VOL: the scene has volumetric
+// objects (otherwise Scene::findInsideObject finds none and is skipped).
+template <bool VOL>
 __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3 n, int hitPos, int insideIn,
                                 bool russian, const float* u, Scat& sc, uint2* lds, const PathBufs& B,
-                                uint32_t& rays) {
+                                uint32_t& rays, bool light = false) {
   const DevObject& O = S.objs[obj];
   sc.x = x;
   sc.contact = obj;
@@ -606,7 +611,7 @@ __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3
   } else if (type == OBJ_REFRACTIVE) {  // RefractiveMaterial::scatter (src/RefractiveMaterial.cpp:132-269)
     sc.random = false;
     int outside = obj;
-    if (insideIn == obj) outside = find_inside(S, x, d, obj, lds, B, rays);
+    if (insideIn == obj) outside = VOL ? find_inside(S, x, d, obj, lds, B, rays) : NONE;
     float n_i = refr_coeff(S, insideIn), n_t = refr_coeff(S, outside);
     Vec3 t;
     float er;
@@ -625,7 +630,7 @@ __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3
       sc.dirProb = p;
     } else {
       sc.dir = t;
-      sc.color = sf * ((n_i * n_i) / (n_t * n_t)) * ((1 - er) / fmaxf(fabsf(dot(n, t)), 1e-6f));
+      sc.color = (light ? sf : sf * ((n_i * n_i) / (n_t * n_t))) * ((1 - er) / fmaxf(fabsf(dot(n, t)), 1e-6f));
       sc.dirProb = 1 - p;
       sc.inside = outside;
     }
@@ -638,7 +643,7 @@ __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3
   }
 }
 
-template <int MODE, bool BATCH>
+template <int MODE, bool BATCH, bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
                                                    BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
@@ -699,7 +704,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, Pass
         bool done = false;
         for (int level = 0; level < 3 && !done; level++) {
           const DevObject& O = S.objs[cur];
-          if (O.type == OBJ_VPM) {  // SceneVPMObject::scatter (src/SceneVPMObject.cpp:4-251)
+          if (VOL && O.type == OBJ_VPM) {  // SceneVPMObject::scatter (src/SceneVPMObject.cpp:4-251)
             const float sampDist = -__logf(1.0f - u[5]) / O.lum_st;
             const bool go_in = (hitObj == cur) && (inside != cur);
             const bool be_in = (inside == cur);
@@ -795,7 +800,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, Pass
               guided_scatter(S.objs[cur], cur, xh, d, hn, pos, inside, russian, head, ug, u[2], sc);
               done = true;
             } else {
-              surface_scatter(S, cur, xh, d, hn, pos, inside, russian, u, sc, lds, B, rays);
+              surface_scatter<VOL>(S, cur, xh, d, hn, pos, inside, russian, u, sc, lds, B, rays);
               done = true;
             }
           }
@@ -934,7 +939,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, Pass
 // crossing is a bounce of the equivalent BSDF-sampled path, so a light reached
 // after c crossings from the vertex of bounce b counts only when
 // b + 1 + c <= max_depth, the paths the PathTracer itself can produce.
-template <bool BATCH>
+template <bool BATCH, bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
                                                     BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
@@ -954,7 +959,7 @@ __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S0, PathBufs B, Pas
   TraceState ts;
   TraceStats st{0u, 0u};
   auto finish = [&](bool visible) {
-    if (visible && inside != NONE && S.objs[inside].type == OBJ_VPM) {
+    if (VOL && visible && inside != NONE && S.objs[inside].type == OBJ_VPM) {
       const DevObject& M = S.objs[inside];
       Vec3 tr = vexp(v3(-M.st[0] * remaining, -M.st[1] * remaining, -M.st[2] * remaining));
       T = T * tr * (__expf(-M.lum_st * remaining) / lum(tr));
@@ -979,7 +984,7 @@ __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S0, PathBufs B, Pas
       return finish(b + 1 + (uint32_t)c <= max_depth);  // no hit at all
     ts.limit = remaining * 1.001f;  // hits beyond the light point cannot block it
     ts.anyhit = remaining;
-    ts.use_anyhit = S.n_vol == 0;  // without media the first blocking hit decides
+    ts.use_anyhit = !VOL || S.n_vol == 0;  // without media the first blocking hit decides
     return true;
   };
   for (;;) {
@@ -1012,7 +1017,7 @@ __global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S0, PathBufs B, Pas
       } else {
         const float hd = ts.best;
         const int hobj = __float_as_int(S.tree.tv1[ts.bestPos].w);
-        if (S.objs[hobj].type != OBJ_VPM) {
+        if (!VOL || S.objs[hobj].type != OBJ_VPM) {
           active = finish(false);
         } else {  // cross an index-matched medium boundary and go on
           const Vec3 xn = o + w * hd;

@@ -22,30 +22,9 @@
 #include <new>
 #include <vector>
 
-#include "rx_pathtrace.h"
+#include "rx_render_state.h"
 
 namespace rx {
-
-// ------------------------------------------------------------------ host state
-struct SceneBufs {
-  TreeBufs global;  // the context's tree is used; this holds volume trees
-  std::vector<TreeBufs*> vols;
-  DBuf vol_trees, vol_obj, objs, etri, enrm, ecdf;
-  DevScene dev{};
-  uint32_t max_depth = 0;
-  std::vector<DevObject> host_objs;
-  ~SceneBufs() {
-    for (auto* v : vols) delete v;
-  }
-};
-
-struct RenderState {
-  SceneBufs scene;
-  DBuf o, d, beta, lastp, L, s, hit, q0, q1, sq, cnt, so, sd, sc, ss, sum, sumsq, mean, stats;
-  DBuf rec, bcount, total;  // guided-mode training records
-  uint32_t cap = 0;
-  float box_lo[3], box_hi[3];
-};
 
 static void leaf_verts_of(const HostTree& t, const float* verts, const uint32_t* tris, std::vector<float>& lv) {
   lv.resize(9 * t.leaf_tris.size());
@@ -164,9 +143,10 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   if (cam->width <= 0 || cam->height <= 0 || (uint64_t)cam->width * cam->height > (1ull << 28))
     return set_err(c, RX_ERR_INVALID, "rx_render: bad resolution");
   if (prm->max_depth >= (uint32_t)kMaxBounce) return set_err(c, RX_ERR_INVALID, "rx_render: max_depth must be < 64");
-  if (prm->mode > RX_MODE_GUIDED) return set_err(c, RX_ERR_INVALID, "rx_render: bad mode");
+  if (prm->mode > RX_MODE_BPT) return set_err(c, RX_ERR_INVALID, "rx_render: bad mode");
   if (prm->mode >= RX_MODE_NEE && c->render->scene.dev.n_etri == 0)
     return set_err(c, RX_ERR_INVALID, "rx_render: next-event estimation needs an emitter");
+  if (prm->mode == RX_MODE_BPT) return render_bpt(c, cam, prm, rgb_mean, sum, sumsq, stats);
   (void)hipSetDevice(c->device);
   RenderState& R = *c->render;
   hipStream_t s = c->stream;
@@ -244,6 +224,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   P.rr_start = prm->rr_start_depth;
   P.mode = prm->mode;
   const DevScene S = R.scene.dev;
+  const bool vol = S.n_vol > 0;  // kernels without the medium code when the scene has no volumes
   const bool timing = (prm->flags & RX_RENDER_TIMING) != 0;
   const bool count = (prm->flags & RX_RENDER_COUNT_WORK) != 0;
   const int mode = (int)prm->mode;
@@ -307,7 +288,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     P.spass = spass;
     done += spass;
     GA.tag = train ? (float)(pass + 1) : 0.0f;
-    hipLaunchKernelGGL((k_generate<false>), dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, BA0);
+    if (vol) hipLaunchKernelGGL((k_generate<false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, BA0);
+    else hipLaunchKernelGGL((k_generate<false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, BA0);
     for (uint32_t b = 0; b <= prm->max_depth; b++) {
       hipEvent_t e0 = timing ? mark() : nullptr;
       if (b == 0) {
@@ -323,16 +305,20 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
         hipLaunchKernelGGL(k_guide_query, dim3(blocks), dim3(kRBlock), 0, s, B, GA, b);
         RXCK(guide_forward(GA.net, GA.X, B.cnt + b, N, nullptr, nullptr, nullptr, c->guide->Z4.as<float>(), c->n_cu, s),
              "guide forward");
-        hipLaunchKernelGGL((k_shade<2, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        if (vol) hipLaunchKernelGGL((k_shade<2, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        else hipLaunchKernelGGL((k_shade<2, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
       } else if (mode == RX_MODE_NEE) {
-        hipLaunchKernelGGL((k_shade<1, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        if (vol) hipLaunchKernelGGL((k_shade<1, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        else hipLaunchKernelGGL((k_shade<1, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
       } else {
-        hipLaunchKernelGGL((k_shade<0, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        if (vol) hipLaunchKernelGGL((k_shade<0, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        else hipLaunchKernelGGL((k_shade<0, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
       }
       hipEvent_t e2 = timing ? mark() : nullptr;
       hipEvent_t e3 = nullptr;
       if (mode) {
-        hipLaunchKernelGGL((k_shadow<false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        if (vol) hipLaunchKernelGGL((k_shadow<false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        else hipLaunchKernelGGL((k_shadow<false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
         e3 = timing ? mark() : nullptr;
       }
       if (timing) {

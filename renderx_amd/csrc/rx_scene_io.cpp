@@ -591,6 +591,7 @@ struct Loader {
       if (r->first("spp")) prm.spp = (uint32_t)atoi(r->first("spp")->value());
     }
     if (type == "PTtest") prm.mode = RX_MODE_NEE;
+    else if (type == "BPT" || type == "BidirectionalPathTracer") prm.mode = RX_MODE_BPT;
     else if (!type.empty() && type != "PathTracer" && type != "PT") {
       err = "renderer " + type + " is not on this path";
       return false;

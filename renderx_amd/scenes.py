@@ -177,6 +177,22 @@ def cornell(width=256, height=256, spp=16, max_depth=8, rr_start_depth=3) -> Sce
     return Scene("cornell", v, t, o, cam, spp, max_depth, rr_start_depth)
 
 
+def lit_plane(width=32, height=32, spp=64, max_depth=8, rr_start_depth=0) -> Scene:
+    """An open scene with a smooth image: a large diffuse floor lit by a quad
+    light beside the view, seen from above.  No edge, emitter or point behind
+    the camera is in view, so every strategy of a bidirectional tracer sees the
+    same smooth radiance field."""
+    b = _Builder()
+    fv, ft = _quad([-3, 0, -3], [-3, 0, 3], [3, 0, 3], [3, 0, -3])
+    b.add(fv, _orient(fv, ft, lambda c: np.tile([0, 1.0, 0], (len(c), 1))), DIFFUSE, (0.5, 0.5, 0.5))
+    h = 0.15
+    qv, qt = _quad([1.0 - h, 0.5, -h], [1.0 + h, 0.5, -h], [1.0 + h, 0.5, h], [1.0 - h, 0.5, h])
+    b.add(qv, _orient(qv, qt, lambda c: np.tile([0, -1.0, 0], (len(c), 1))), EMISSIVE, (20, 20, 20))
+    v, t, o = b.finish()
+    cam = make_camera([0, 1.2, 0], [0, 0, 0], [0, 0, 1], width, height)
+    return Scene("lit_plane", v, t, o, cam, spp, max_depth, rr_start_depth)
+
+
 # ---------------------------------------------------------------- geodesic spheres
 def icosphere(freq: int):
     """Class-I geodesic sphere: 20*freq^2 triangles, 10*freq^2+2 shared vertices, unit radius."""
@@ -301,6 +317,26 @@ def media_scene(width=512, height=512, spp=256, max_depth=16, freq=158, glass_fr
     v, t, o = b.finish()
     cam = make_camera([0.5, 0.5, -1.374], [0.5, 0.5, 0.5], [0, 1, 0], width, height)
     return Scene("media", v, t, o, cam, spp, max_depth, 0, {"freq": freq})
+
+
+def glass_scene(width=64, height=64, spp=64, max_depth=10, glass_freq=8, phong_freq=8, phong=True) -> Scene:
+    """The Cornell box with a glass sphere (IOR 1.5, refraction both ways,
+    caustics) and a Phong sphere (a diffuse one with phong=False), without a
+    medium: a surface scene for the bidirectional tracer."""
+    b = _Builder()
+    _cornell(b)
+    gv, gt = icosphere(glass_freq)
+    gv = gv * 0.15 + np.array([0.35, 0.451, 0.38])
+    b.add(gv, gt, REFRACTIVE, (1, 1, 1), 0.0, (1, 1, 1), 1.5)
+    dv, dt = icosphere(phong_freq)
+    dv = dv * 0.2 + np.array([0.76, 0.215, 0.22])
+    if phong:
+        b.add(dv, dt, PHONG, (0.5, 0.5, 0.5), 50.0, (0.3, 0.3, 0.3))
+    else:
+        b.add(dv, dt, DIFFUSE, (0.5, 0.5, 0.5))
+    v, t, o = b.finish()
+    cam = make_camera([0.5, 0.5, -1.374], [0.5, 0.5, 0.5], [0, 1, 0], width, height)
+    return Scene("glass", v, t, o, cam, spp, max_depth, 0)
 
 
 def soup_scene(index: int, n_tris: int | None = None) -> Scene:

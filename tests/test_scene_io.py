@@ -162,6 +162,13 @@ def test_rejections(tmp_path, material, msg):
         api.load_scene_config(str(tmp_path / "config.xml"))
 
 
+@pytest.mark.parametrize("name", ["BPT", "BidirectionalPathTracer"])
+def test_bidirectional_renderer_name(tmp_path, name):
+    cfg = _write_scene(tmp_path, scenes.cornell(8, 8), renderer=name, max_depth=7)
+    ld = api.load_scene_config(cfg)
+    assert ld.meta["mode"] == api.MODE_BPT and ld.max_depth == 7
+
+
 def test_unknown_renderer_and_missing_file(tmp_path):
     with pytest.raises(api.RxError):
         api.load_scene_config(str(tmp_path / "nope.xml"))
