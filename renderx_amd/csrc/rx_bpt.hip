@@ -257,8 +257,7 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_pairs(DevScene S, PathBufs B, B
   const uint32_t K = A.K, total = A.N * K;
   for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < total; i0 += stride) {
     const uint32_t i = i0 + threadIdx.x;
-    uint32_t p = 0, j = 0, nL = 0;
-    bool eye_ok = false;
+    uint32_t p = 0, j = 0, mask = 0;
     if (i < total) {
       p = i / K;
       j = i - p * K;
@@ -266,21 +265,19 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_pairs(DevScene S, PathBufs B, B
         int ce;
         bool re;
         vert_flags(A.V, (size_t)p * K + j, ce, re);
-        eye_ok = re && !is_emissive(S, ce);
-        nL = A.nv[A.N + p];
+        if (re && !is_emissive(S, ce)) {
+          const uint32_t nL = A.nv[A.N + p];
+          for (uint32_t k = 0; k < nL; k++) {
+            int cl;
+            bool rl;
+            vert_flags(A.V, (size_t)(A.N + p) * K + k, cl, rl);
+            if (rl && !(k >= 1 && is_emissive(S, cl))) mask |= 1u << k;
+          }
+        }
       }
     }
-    for (uint32_t k = 0; k < K; k++) {
-      bool ok = false;
-      if (eye_ok && k < nL) {
-        int cl;
-        bool rl;
-        vert_flags(A.V, (size_t)(A.N + p) * K + k, cl, rl);
-        ok = rl && !(k >= 1 && is_emissive(S, cl));
-      }
-      const uint32_t slot = wave_append(&B.cnt[kBptPairCount], ok);
-      if (ok) A.pairs[slot] = (p << 10) | (j << 5) | k;
-    }
+    uint32_t slot = wave_append_n(&B.cnt[kBptPairCount], (uint32_t)__popc(mask));
+    for (; mask; mask &= mask - 1) A.pairs[slot++] = (p << 10) | (j << 5) | (uint32_t)(__ffs(mask) - 1);
   }
 }
 

@@ -266,6 +266,21 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
   base = __shfl(base, leader);
   return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
+// Appends c entries per lane with one atomic per wave; returns the lane's
+// first slot (lanes in order).
+__device__ __forceinline__ uint32_t wave_append_n(uint32_t* counter, uint32_t c) {
+  const int lane = threadIdx.x & 63;
+  uint32_t x = c;  // inclusive prefix sum over the wave
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  const uint32_t total = __shfl(x, 63);
+  uint32_t base = 0;
+  if (lane == 0 && total) base = atomicAdd(counter, total);
+  base = __shfl(base, 0);
+  return base + x - c;
+}
 __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
