@@ -80,11 +80,16 @@ def cpu_baseline(scene, seconds_target=15.0):
                       f"{scene.max_depth} ({st['rays']} traversal queries in {dt:.1f} s)"}
 
 
-def equal_time_relmse(res=256, ref_spp=8192, budget_s=5.0, chunk=16):
+def equal_time_relmse(res=256, ref_spp=32768, budget_s=5.0, chunk=64):
     """The metric's second half on the media + glass scene (homogeneous
     medium box, glass sphere, Phong displaced sphere in a Cornell box): relMSE
     against a converged image (NEE, ref_spp samples) of the plain path tracer
-    and the guided renderer (training included) after the same wall time."""
+    and the guided renderer (training included) after the same wall time.
+    Each render call adds `chunk` samples per pixel (one pass of every
+    renderer), like a progressive render between two snapshots; a renderer
+    keeps going until the budget has passed, and its relMSE is scaled by
+    elapsed / budget (Monte Carlo error falls as 1 / time), so that the
+    renderers whose calls take longer gain nothing from overshooting."""
     from renderx_amd import api, scenes, film
 
     sc = scenes.media_scene(res, res, max_depth=16, freq=158, glass_freq=16)
@@ -105,10 +110,14 @@ def equal_time_relmse(res=256, ref_spp=8192, budget_s=5.0, chunk=16):
             r = pt.renderPixels(spp=chunk, mode=mode, seed=5, sample_offset=n)
             acc = r["sum"] if acc is None else acc + r["sum"]
             n += chunk
+        el = time.perf_counter() - t0
         img = (acc / n).astype(np.float32)
-        out[name] = film.relmse(img, ref)
+        raw = film.relmse(img, ref)
+        out[name] = raw * el / budget_s
+        out[name + "_raw"] = raw
+        out[name + "_s"] = el
         out[name + "_spp"] = n
-        log(f"[bench] {name}: {n} spp in {time.perf_counter() - t0:.1f} s, relMSE {out[name]:.5f}")
+        log(f"[bench] {name}: {n} spp in {el:.2f} s, relMSE {raw:.5f} ({out[name]:.5f} at {budget_s:.0f} s)")
     out["plain_over_guided"] = out["plain"] / out["guided"]
     return out
 

@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_record(DevScene S, PathBufs B, 
         int cur = obj;
         if (S.objs[cur].type == OBJ_REFRACTIVE && hitObj != cur && inside == cur) cur = hitObj;  // delegation
         if (cur != NONE && pos >= 0)
-          surface_scatter<false>(S, cur, o + d * t, d, hn, pos, inside, russian, u, sc, lds, B, rays, light);
+          surface_scatter(S, cur, o + d * t, d, hn, pos, inside, russian, u, sc, NONE, light);
         Vec3 nc = v3(0, 0, 0);
         if (sc.contact >= 0 && sc.contactPos >= 0) nc = xyz(S.tree.tn[sc.contactPos]);
         if (length(sc.dir) >= 0.5f && sc.inside == NONE && sc.contact != NONE) {  // light-leak guard (:183-201)

@@ -19,44 +19,45 @@ struct GuideNet {
   float lo[3], hi[3];     // encoding bounds (scene box)
 };
 
-// Decoded head of one row: mixture weights, unit means, concentrations, alpha.
+// Head of one row of the network output z (48 floats): the softmax shift and
+// scale of the mixture weights and the selection probability alpha are kept,
+// each lobe (weight, unit mean, concentration) is decoded where it is used,
+// so a renderer thread holds 4 values instead of the 49 of a decoded head.
 struct GuideHead {
-  float pi[kGLobes];
-  float mu[kGLobes][3];
-  float kappa[kGLobes];
-  float norm[kGLobes];  // kappa / (2 pi (1 - exp(-2 kappa)))
-  float alpha;
+  const float* z;
+  float mx, inv, alpha;
 };
 
 __device__ __forceinline__ void guide_head(const float* z, GuideHead& h) {
   float mx = z[0];
   for (int j = 1; j < kGLobes; j++) mx = fmaxf(mx, z[5 * j]);
   float s = 0.0f;
-  for (int j = 0; j < kGLobes; j++) {
-    h.pi[j] = __expf(z[5 * j] - mx);
-    s += h.pi[j];
-  }
-  const float inv = 1.0f / s;
-  for (int j = 0; j < kGLobes; j++) {
-    h.pi[j] *= inv;
-    float m0 = z[5 * j + 1], m1 = z[5 * j + 2], m2 = z[5 * j + 3];
-    float l = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
-    h.mu[j][0] = m0 / l;
-    h.mu[j][1] = m1 / l;
-    h.mu[j][2] = m2 / l;
-    float lk = fminf(8.0f, fmaxf(-3.0f, z[5 * j + 4]));
-    float k = __expf(lk);
-    h.kappa[j] = k;
-    h.norm[j] = k / (6.28318530717958647692f * (1.0f - __expf(-2.0f * k)));
-  }
+  for (int j = 0; j < kGLobes; j++) s += __expf(z[5 * j] - mx);
+  h.z = z;
+  h.mx = mx;
+  h.inv = 1.0f / s;
   h.alpha = 0.05f + 0.9f / (1.0f + __expf(-z[40]));
+}
+__device__ __forceinline__ float guide_weight(const GuideHead& h, int j) { return __expf(h.z[5 * j] - h.mx) * h.inv; }
+__device__ __forceinline__ void guide_lobe(const GuideHead& h, int j, float mu[3], float& kappa) {
+  const float* z = h.z + 5 * j;
+  float m0 = z[1], m1 = z[2], m2 = z[3];
+  float l = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
+  mu[0] = m0 / l;
+  mu[1] = m1 / l;
+  mu[2] = m2 / l;
+  float lk = fminf(8.0f, fmaxf(-3.0f, z[4]));
+  kappa = __expf(lk);
 }
 
 __device__ __forceinline__ float guide_pdf(const GuideHead& h, Vec3 w) {
   float q = 0.0f;
   for (int j = 0; j < kGLobes; j++) {
-    float c = h.mu[j][0] * w.x + h.mu[j][1] * w.y + h.mu[j][2] * w.z;
-    q += h.pi[j] * h.norm[j] * __expf(h.kappa[j] * (c - 1.0f));
+    float mu[3], k;
+    guide_lobe(h, j, mu, k);
+    const float norm = k / (6.28318530717958647692f * (1.0f - __expf(-2.0f * k)));  // kappa / (2 pi (1 - e^-2kappa))
+    float c = mu[0] * w.x + mu[1] * w.y + mu[2] * w.z;
+    q += guide_weight(h, j) * norm * __expf(k * (c - 1.0f));
   }
   return q;
 }
@@ -65,18 +66,19 @@ __device__ __forceinline__ float guide_pdf(const GuideHead& h, Vec3 w) {
 // "Numerically stable sampling of the von Mises Fisher distribution on S^2", 2012).
 __device__ __forceinline__ Vec3 guide_sample(const GuideHead& h, float u0, float u1, float u2) {
   int j = 0;
-  float c = h.pi[0];
+  float c = guide_weight(h, 0);
   while (j < kGLobes - 1 && u0 >= c) {
     j++;
-    c += h.pi[j];
+    c += guide_weight(h, j);
   }
-  const float k = h.kappa[j];
+  float mu[3], k;
+  guide_lobe(h, j, mu, k);
   float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * k)) / k;
   w = fminf(1.0f, fmaxf(-1.0f, w));
   float r = sqrtf(fmaxf(0.0f, 1.0f - w * w));
   float sp, cp;
   __sincosf(6.28318530717958647692f * u2, &sp, &cp);
-  Vec3 m = v3(h.mu[j][0], h.mu[j][1], h.mu[j][2]);
+  Vec3 m = v3(mu[0], mu[1], mu[2]);
   Vec3 tmp = fabsf(m.x) > 0.9f ? v3(0, 1, 0) : v3(1, 0, 0);
   Vec3 t1 = normalize(cross(m, tmp));
   Vec3 t2 = cross(m, t1);

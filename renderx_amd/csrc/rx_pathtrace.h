@@ -27,6 +27,10 @@ constexpr int kRBlock = kTraceBlock;  // render kernels share the traversal bloc
 #endif
 // minimum resident waves per SIMD the extend kernel's registers must allow
 constexpr int kExtendMinWaves = RX_EXTEND_MIN_WAVES;
+#ifndef RX_SHADOW_MIN_WAVES
+#define RX_SHADOW_MIN_WAVES 1
+#endif
+constexpr int kShadowMinWaves = RX_SHADOW_MIN_WAVES;  // the same for the shadow kernel
 constexpr int kMaxBounce = 64;
 constexpr int kShadowBase = 96;       // cnt[kShadowBase + b]: shadow queue length of bounce b
 constexpr int kWorkBase = 192;       // cnt[kWorkBase + b]: extend work counter of bounce b
@@ -556,13 +560,12 @@ __device__ __forceinline__ void surface_bsdf(const DevObject& O, Vec3 n, Vec3 d,
 }
 
 // One scatter of a surface object (the Material::scatter implementations).
+// exit_outside: the object the ray is in once it leaves obj
+// (Scene::findInsideObject, computed by the caller when insideIn == obj).
 // light: the scatter of a light subpath (fixIsLight), which leaves out the
-// refracted radiance's (n_i/n_t)^2 scaling.  VOL: the scene has volumetric
-// objects (otherwise Scene::findInsideObject finds none and is skipped).
-template <bool VOL>
+// refracted radiance's (n_i/n_t)^2 scaling.
 __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3 n, int hitPos, int insideIn,
-                                bool russian, const float* u, Scat& sc, uint2* lds, const PathBufs& B,
-                                uint32_t& rays, bool light = false) {
+                                bool russian, const float* u, Scat& sc, int exit_outside, bool light = false) {
   const DevObject& O = S.objs[obj];
   sc.x = x;
   sc.contact = obj;
@@ -626,7 +629,7 @@ __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3
   } else if (type == OBJ_REFRACTIVE) {  // RefractiveMaterial::scatter (src/RefractiveMaterial.cpp:132-269)
     sc.random = false;
     int outside = obj;
-    if (insideIn == obj) outside = VOL ? find_inside(S, x, d, obj, lds, B, rays) : NONE;
+    if (insideIn == obj) outside = exit_outside;
     float n_i = refr_coeff(S, insideIn), n_t = refr_coeff(S, outside);
     Vec3 t;
     float er;
@@ -714,6 +717,15 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, Pass
         sc.pmix = sc.pb = sc.qrr = 0.0f;
         GuideHead head;
         const Vec3 xh = o + d * t;  // hit point (valid when pos >= 0)
+        // the one containment query a scatter can need: leaving the object
+        // the path is in through its own boundary (a refractive object, or a
+        // medium whose sampled distance passes the boundary)
+        int exit_outside = NONE;
+        if (VOL && pos >= 0 && inside != NONE && hitObj == inside) {
+          const DevObject& I = S.objs[inside];
+          if (I.type == OBJ_REFRACTIVE || (I.type == OBJ_VPM && -__logf(1.0f - u[5]) / I.lum_st >= t))
+            exit_outside = find_inside(S, xh, d, inside, lds, B, rays);
+        }
         int cur = obj;
         float originScale = 1.0f;
         bool done = false;
@@ -726,7 +738,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, Pass
             const bool out_of_vol = !(pos >= 0) || sampDist >= t;
             if (go_in || (be_in && out_of_vol && hitObj == cur)) {
               // cross the own boundary, entering or leaving
-              int outside = go_in ? cur : find_inside(S, xh, d, cur, lds, B, rays);
+              int outside = go_in ? cur : exit_outside;
               float n_i = go_in ? refr_coeff(S, inside) : O.ior, n_t = refr_coeff(S, outside);
               sc.x = xh;
               sc.contact = cur;
@@ -815,7 +827,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, Pass
               guided_scatter(S.objs[cur], cur, xh, d, hn, pos, inside, russian, head, ug, u[2], sc);
               done = true;
             } else {
-              surface_scatter<VOL>(S, cur, xh, d, hn, pos, inside, russian, u, sc, lds, B, rays);
+              surface_scatter(S, cur, xh, d, hn, pos, inside, russian, u, sc, exit_outside);
               done = true;
             }
           }
@@ -955,7 +967,7 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, Pass
 // after c crossings from the vertex of bounce b counts only when
 // b + 1 + c <= max_depth, the paths the PathTracer itself can produce.
 template <bool BATCH, bool VOL>
-__global__ void __launch_bounds__(kRBlock) k_shadow(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
+__global__ void __launch_bounds__(kRBlock, kShadowMinWaves) k_shadow(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
                                                     BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
   const uint32_t n = B.cnt[kShadowBase + b];

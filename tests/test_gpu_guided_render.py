@@ -51,6 +51,8 @@ CHUNK = 256
 
 
 def _equal_time(pt, mode, budget_s, chunk, seed):
+    """Renders until the budget has passed; the caller scales the error by
+    elapsed / budget (Monte Carlo error falls as 1 / time)."""
     s = None
     q = None
     n = 0
@@ -89,7 +91,7 @@ def test_equal_time_relmse_guided_vs_plain(rx, gpu_ctx):
     for name, mode in [("plain", MODE_PT), ("nee", MODE_NEE), ("guided", MODE_GUIDED)]:
         pt = _guided(rx, gpu_ctx, sc) if mode == MODE_GUIDED else rx.PathTracerHIP(sc, ctx=gpu_ctx)
         img, n, dt = _equal_time(pt, mode, budget, CHUNK, seed=5)
-        res[name] = rel_mse(img, ref)
+        res[name] = rel_mse(img, ref) * dt / budget
         print(f"  {name:7s}: {n:6d} spp in {dt:.2f} s, relMSE {res[name]:.5f}")
     print(f"  guided vs plain: {res['plain'] / res['guided']:.2f}x, guided vs NEE-only: {res['nee'] / res['guided']:.2f}x")
     assert res["guided"] * 2 <= res["plain"]
@@ -106,7 +108,7 @@ def test_trained_network_beats_untrained(rx, gpu_ctx):
     for name, passes in [("untrained", 0), ("trained", 16)]:
         pt = _guided(rx, gpu_ctx, sc, passes=passes)
         img, n, dt = _equal_time(pt, MODE_GUIDED, 2.0, CHUNK, seed=5)
-        res[name] = rel_mse(img, ref)
+        res[name] = rel_mse(img, ref) * dt / 2.0
         print(f"  {name:9s}: {n:6d} spp in {dt:.2f} s, relMSE {res[name]:.5f}")
     print(f"  network gain {res['untrained'] / res['trained']:.2f}x")
     assert res["trained"] < res["untrained"]

@@ -99,13 +99,15 @@ def test_image_matches_oracle_pathtracer(rx, oracle_mod, gpu_ctx, name, mode):
 
 def test_cornell_nee_matches_pathtracertest(rx, oracle_mod, gpu_ctx):
     """On the all-diffuse Cornell box PathTracerTest (NEE+MIS) is consistent with
-    PathTracer up to the camera cosine/vignetting factor (~1e-4 here)."""
+    PathTracer up to the camera cosine/vignetting factor (~1e-4 here).  The
+    oracle's NEE samples are heavy-tailed (light points close to a wall), so
+    it takes 16384 of them per pixel for a reliable image-mean comparison."""
     print("[render parity] cornell NEE vs PathTracerTest", flush=True)
     sc = scenes.cornell(RES, RES, spp=16, max_depth=8, rr_start_depth=0)
     so = oracle_mod.SceneOracle(sc)
-    osum, osq = so.render(8192, seed=6, mode=1)
+    osum, osq = so.render(16384, seed=6, mode=1)
     r = _render(rx, gpu_ctx, sc, GPU_SPP, MODE_NEE, rr_start_depth=0)
-    _check("cornell NEE vs PTtest", r["sum"], r["sumsq"], GPU_SPP, osum, osq, 8192, pooled=False)
+    _check("cornell NEE vs PTtest", r["sum"], r["sumsq"], GPU_SPP, osum, osq, 16384, pooled=False)
 
 
 def test_render_is_deterministic_and_resumable(rx, gpu_ctx):
