@@ -238,8 +238,10 @@ struct TraceState {
   bool fast, has;
   float best;
   float limit;  // subtrees whose entry key exceeds it are culled (INF: none)
-  float anyhit; // with use_anyhit, a legal hit with dist - anyhit < -1e-5 ends the traversal
+  float anyhit; // with use_anyhit, a legal hit with dist - anyhit < -1e-5 ends the traversal,
   bool use_anyhit;
+  bool anyhit_front;     // ... and with anyhit_front only a front-facing hit on an object other
+  int32_t anyhit_skip;   // than anyhit_skip does
   int32_t bestLeaf, bestPos, ref, inside, contact;
   int sp;
 };
@@ -267,6 +269,8 @@ __device__ __forceinline__ bool trace_begin(const TreeRoot& R, TraceState& s, Ve
   s.limit = __builtin_inff();
   s.anyhit = 0.0f;
   s.use_anyhit = false;
+  s.anyhit_front = false;
+  s.anyhit_skip = INT32_MIN;
   s.bestLeaf = s.bestPos = -1;
   s.sp = 0;
   s.ref = R.ref;
@@ -288,7 +292,7 @@ __device__ __forceinline__ bool trace_begin(const TreeRoot& R, TraceState& s, Ve
 //
 // lds_stack: per-lane LDS column of LDS_DEPTH entries (stride = blockDim.x),
 // spill: per-thread global overflow of spill_depth entries.
-template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH>
+template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, bool ANYHIT = true>
 __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint2* lds_stack, uint2* spill,
                                            int spill_depth, TraceStats* st, bool* overflow) {
   const float INF = __builtin_inff();
@@ -306,8 +310,11 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
     if (s.fast) {  // the exact predicate only where the slab filter cannot decide
       const int cl = slab_classify(llo, lhi, s.o, s.inv, s.beta, tl);
       const int cr = slab_classify(rlo, rhi, s.o, s.inv, s.beta, tr);
-      hl = cl == kBoxHit || (cl == kBoxUnknown && box_hit(llo, lhi, s.o, s.d));
-      hr = cr == kBoxHit || (cr == kBoxUnknown && box_hit(rlo, rhi, s.o, s.d));
+      // a child the ordered traversal culls (entry beyond the cull distance)
+      // needs no exact test: its answer would not be used
+      const float cull = EXHAUSTIVE ? INF : fminf(s.has ? s.best * 1.001f : INF, s.limit);
+      hl = cl == kBoxHit || (cl == kBoxUnknown && (EXHAUSTIVE || tl <= cull) && box_hit(llo, lhi, s.o, s.d));
+      hr = cr == kBoxHit || (cr == kBoxUnknown && (EXHAUSTIVE || tr <= cull) && box_hit(rlo, rhi, s.o, s.d));
     } else {
       hl = box_hit(llo, lhi, s.o, s.d);
       hr = box_hit(rlo, rhi, s.o, s.d);
@@ -355,16 +362,18 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
       if (!(dist >= 0)) continue;
       bool win = !s.has || dist < s.best || (dist == s.best && leaf > s.bestLeaf);
       if (!win) continue;
+      float4 n = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (LEGALITY) {
-        float4 n = T.tn[p];
+        n = T.tn[p];
         if (!legal(s.d, xyz(n), __float_as_int(b.w), s.inside, s.contact)) continue;
       }
       s.has = true;
       s.best = dist;
       s.bestLeaf = leaf;
       s.bestPos = p;
-      if (s.use_anyhit && dist - s.anyhit < -1e-5f) {  // the caller only needs to know that such a hit exists
-        s.ref = INT32_MIN;
+      if (ANYHIT && s.use_anyhit && dist - s.anyhit < -1e-5f &&
+          (!s.anyhit_front || (__float_as_int(b.w) != s.anyhit_skip && dot(s.d, xyz(n)) < 0.0f))) {
+        s.ref = INT32_MIN;  // the caller only needs to know that such a hit exists
         return true;
       }
     }
@@ -484,7 +493,7 @@ __device__ __forceinline__ void trace_range(const DevTree& T, uint32_t n, uint32
     }
     if (w.drained && !__any(active)) break;
     if (active &&
-        trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH>(T, ts, lds_stack, spill, spill_depth, &st, overflow)) {
+        trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH, false>(T, ts, lds_stack, spill, spill_depth, &st, overflow)) {
       emit(cur, ts.has ? Hit{ts.best, ts.bestPos} : Hit{-1.0f, -1}, st);
       active = false;
     }

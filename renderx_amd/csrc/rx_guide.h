@@ -94,28 +94,32 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Layout of the 96 values: (sin, cos) pairs, octave-major: position octave k
+// of axis a at 6k + 2a, direction octave k of axis a at 72 + 6k + 2a.  out is
+// 16-byte aligned (rows of 192 bytes); the row is written as 12 vector stores.
 __device__ __forceinline__ void guide_encode(const GuideNet& N, Vec3 p, Vec3 d, uint16_t* out) {
   const float pc[3] = {p.x, p.y, p.z}, dc[3] = {d.x, d.y, d.z};
+  uint32_t w[kGIn / 2];  // (sin, cos) bf16 pairs
   for (int a = 0; a < 3; a++) {
     float x = (pc[a] - N.lo[a]) / (N.hi[a] - N.lo[a]);
     x = fminf(1.0f, fmaxf(0.0f, x));
     float f = x;
     for (int k = 0; k < 12; k++) {
-      float s, c;
-      sincospif(f, &s, &c);
-      out[6 * k + 2 * a] = f2bf(s);
-      out[6 * k + 2 * a + 1] = f2bf(c);
+      float sn, cs;
+      sincospif(f, &sn, &cs);
+      w[3 * k + a] = (uint32_t)f2bf(sn) | ((uint32_t)f2bf(cs) << 16);
       f *= 2.0f;
     }
     f = dc[a];
     for (int k = 0; k < 4; k++) {
-      float s, c;
-      sincospif(f, &s, &c);
-      out[72 + 6 * k + 2 * a] = f2bf(s);
-      out[72 + 6 * k + 2 * a + 1] = f2bf(c);
+      float sn, cs;
+      sincospif(f, &sn, &cs);
+      w[36 + 3 * k + a] = (uint32_t)f2bf(sn) | ((uint32_t)f2bf(cs) << 16);
       f *= 2.0f;
     }
   }
+  uint4* o4 = reinterpret_cast<uint4*>(out);
+  for (int v = 0; v < kGIn / 8; v++) o4[v] = make_uint4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
 }
 
 // Launchers (rx_guide.hip).  `rows` points at a device row count; the

@@ -43,12 +43,32 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
 
 // ------------------------------------------------------------------ forward
 // X [M x 96] bf16 -> A1..A3 [M x 64] bf16 (post-ReLU), Z4 [M x 48] fp32.
+// Each block stages the four weight matrices in LDS once (rows padded by 8
+// bf16 so that a fragment load's 16 rows fall in different banks), then its
+// waves run 16-row tiles: the weights are read from HBM/L2 once per block
+// instead of once per tile.
+constexpr int kWPad = 8;
+constexpr int kWOff[4] = {0, kGHid * (kGIn + kWPad), kGHid * (kGIn + kWPad) + kGHid * (kGHid + kWPad),
+                          kGHid * (kGIn + kWPad) + 2 * kGHid * (kGHid + kWPad)};
+constexpr int kWLds = kWOff[3] + kGOut * (kGHid + kWPad);  // bf16 values
+
 __global__ void __launch_bounds__(256) k_mlp_fwd(GuideNet N, const uint16_t* X, const uint32_t* rows, uint32_t cap,
                                                  uint16_t* A1, uint16_t* A2, uint16_t* A3, float* Z4) {
   __shared__ __attribute__((aligned(16))) uint16_t act[2][4][16 * 64];
+  __shared__ __attribute__((aligned(16))) uint16_t wts[kWLds];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
   const uint32_t M = min(*rows, cap);
   const uint32_t nTiles = (M + 15) / 16;
+  if (blockIdx.x * 4 >= nTiles) return;  // no tile for this block (uniform)
+  for (int L = 0; L < 4; L++) {
+    const int K = kLayerIn[L], NO = kLayerOut[L];
+    for (int e = threadIdx.x; e < NO * K / 4; e += blockDim.x) {  // 4 bf16 per step
+      const int r = (4 * e) / K, col = (4 * e) % K;
+      *reinterpret_cast<uint2*>(&wts[kWOff[L] + r * (K + kWPad) + col]) =
+          *reinterpret_cast<const uint2*>(N.Wt[L] + (size_t)r * K + col);
+    }
+  }
+  __syncthreads();
   uint16_t* const outs[3] = {A1, A2, A3};
   for (uint32_t tb = blockIdx.x * 4; tb < nTiles; tb += gridDim.x * 4) {
     const uint32_t tile = tb + wave;
@@ -60,11 +80,12 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(GuideNet N, const uint16_t* X, 
       v4f acc[4];
       for (int nt = 0; nt < 4; nt++) acc[nt] = v4f{0, 0, 0, 0};
       const uint16_t* ain = (L == 0) ? nullptr : act[(L - 1) & 1][wave];
+      const uint16_t* wl = wts + kWOff[L];
       for (int ks = 0; ks < K / 16; ks++) {
         v4s a = live ? ((L == 0) ? ld4(X + (size_t)rowA * kGIn + ks * 16 + 4 * lg) : ld4(ain + lr * 64 + ks * 16 + 4 * lg))
                      : v4s{0, 0, 0, 0};
         for (int nt = 0; nt < NT; nt++) {
-          v4s b = ld4(N.Wt[L] + (size_t)(nt * 16 + lr) * K + ks * 16 + 4 * lg);
+          v4s b = ld4(wl + (nt * 16 + lr) * (K + kWPad) + ks * 16 + 4 * lg);
           acc[nt] = mfma(a, b, acc[nt]);
         }
       }

@@ -35,7 +35,8 @@ constexpr int kMaxBounce = 64;
 constexpr int kShadowBase = 96;       // cnt[kShadowBase + b]: shadow queue length of bounce b
 constexpr int kWorkBase = 192;       // cnt[kWorkBase + b]: extend work counter of bounce b
 constexpr int kShadowWorkBase = 256; // cnt[kShadowWorkBase + b]: shadow work counter of bounce b
-constexpr int kCntWords = 324;
+constexpr int kGuideRowBase = 324;   // cnt[kGuideRowBase + b]: network rows of bounce b (guided mode)
+constexpr int kCntWords = kGuideRowBase + kMaxBounce;
 constexpr uint32_t kDefaultPathSlots = 1u << 25;  // rx_render_tuning.path_slots default (~5.7 GB of path state)
 constexpr uint32_t kGuidedPathSlots = 1u << 22;   // guided mode: the network's per-row buffers scale with the slots
 constexpr int kMaxCross = 4;          // index-matched boundary crossings per shadow ray
@@ -166,10 +167,12 @@ struct GuideArgs {
   GuideNet net;
   const float* Z4;
   uint16_t* X;
+  uint32_t* row;  // network row of each queue entry of the bounce (kNoRow: not guided)
   float* rec;
   uint32_t R;
   float tag;  // > 0: record this pass with that tag
 };
+constexpr uint32_t kNoRow = 0xffffffffu;
 constexpr int kRecFloats = 16;
 
 // ------------------------------------------------------------------ helpers
@@ -431,16 +434,71 @@ __global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend(DevTree T, 
 // ------------------------------------------------------------------ guide queries
 // Encode (hit point, view direction) of every queued path of bounce b; row i
 // of the network batch belongs to queue entry i.
-static __global__ void __launch_bounds__(kRBlock) k_guide_query(PathBufs B, GuideArgs GA, uint32_t b) {
+// Whether the shade kernel's scatter at this hit will be a guided surface
+// scatter: the object resolution of its level loop (media, delegation from
+// a refractive inside object) without the scatter itself.
+template <bool VOL>
+__device__ __forceinline__ bool guided_vertex(const DevScene& S, int inside, int hitObj, int pos, float t, Vec3 d,
+                                              Vec3 hn, float u5) {
+  int cur = inside != NONE ? inside : hitObj;
+  if (cur == NONE) return false;
+  for (int level = 0; level < 3; level++) {
+    const DevObject& O = S.objs[cur];
+    if (VOL && O.type == OBJ_VPM) {
+      const float sampDist = -__logf(1.0f - u5) / O.lum_st;
+      const bool be_in = (inside == cur);
+      const bool out_of_vol = !(pos >= 0) || sampDist >= t;
+      if (be_in && out_of_vol && pos >= 0 && hitObj != cur) {  // a surface inside the medium
+        cur = hitObj;
+        continue;
+      }
+      return false;  // boundary crossing, medium scatter or nothing
+    }
+    if (O.type == OBJ_REFRACTIVE && hitObj != cur && inside == cur) {
+      if (pos < 0) return false;
+      cur = hitObj;
+      continue;
+    }
+    if (pos < 0) return false;
+    return dot(d, hn) <= 0 && (O.type == OBJ_DIFFUSE || O.type == OBJ_GLOSSY || O.type == OBJ_PHONG);
+  }
+  return false;
+}
+
+// Guided mode: the network's inputs for the bounce's guided vertices only,
+// compacted into rows [0, cnt[kGuideRowBase + b]); row[i] maps queue entry i
+// to its row.
+template <bool VOL>
+__global__ void __launch_bounds__(kRBlock) k_guide_query(DevScene S, PathBufs B, PassArgs P, GuideArgs GA, uint32_t b) {
   const uint32_t n = B.cnt[b];
   const uint32_t* q = (b & 1) ? B.q1 : B.q0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t p = q[i];
-    const float4 o = B.o[p], d = B.d[p];
-    const int2 h = B.hit[p];
-    const float t = h.y >= 0 ? __int_as_float(h.x) : 0.0f;
-    Vec3 dd = xyz(d);
-    guide_encode(GA.net, xyz(o) + dd * t, -dd, GA.X + (size_t)i * kGIn);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+    const uint32_t i = i0 + threadIdx.x;
+    bool want = false;
+    Vec3 x = v3(0, 0, 0), dd = v3(0, 0, 0);
+    if (i < n) {
+      const uint32_t p = q[i];
+      uint32_t pix, smp;
+      slot_pixel(P, p, pix, smp);
+      const float4 o = B.o[p], d = B.d[p];
+      const int2 h = B.hit[p];
+      const int pos = h.y;
+      const float t = __int_as_float(h.x);
+      int hitObj = NONE;
+      Vec3 hn = v3(0, 0, 0);
+      if (pos >= 0) {
+        hitObj = __float_as_int(S.tree.tv1[pos].w);
+        hn = xyz(S.tree.tn[pos]);
+      }
+      dd = xyz(d);
+      const float u5 = VOL ? u01(philox(U4{smp, b, 1u, 0u}, pix, P.key1).y) : 0.0f;
+      want = guided_vertex<VOL>(S, B.s[p].x, hitObj, pos, t, dd, hn, u5);
+      x = xyz(o) + dd * (pos >= 0 ? t : 0.0f);
+    }
+    const uint32_t r = wave_append(&B.cnt[kGuideRowBase + b], want);
+    if (i < n) GA.row[i] = want ? r : kNoRow;
+    if (want) guide_encode(GA.net, x, -dd, GA.X + (size_t)r * kGIn);
   }
 }
 
@@ -820,8 +878,8 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, Pass
               done = true;
             } else if (MODE == 2 && dot(d, hn) <= 0 &&
                        (S.objs[cur].type == OBJ_DIFFUSE || S.objs[cur].type == OBJ_GLOSSY ||
-                        S.objs[cur].type == OBJ_PHONG)) {
-              guide_head(GA.Z4 + (size_t)i * kGOut, head);
+                        S.objs[cur].type == OBJ_PHONG) && GA.row[i] != kNoRow) {
+              guide_head(GA.Z4 + (size_t)GA.row[i] * kGOut, head);
               U4 r3 = philox(U4{smp, b, 3u, 0u}, pix, pc.key1);
               const float ug[4] = {u01(r3.x), u01(r3.y), u01(r3.z), u01(r3.w)};
               guided_scatter(S.objs[cur], cur, xh, d, hn, pos, inside, russian, head, ug, u[2], sc);
@@ -1011,7 +1069,16 @@ __global__ void __launch_bounds__(kRBlock, kShadowMinWaves) k_shadow(DevScene S0
       return finish(b + 1 + (uint32_t)c <= max_depth);  // no hit at all
     ts.limit = remaining * 1.001f;  // hits beyond the light point cannot block it
     ts.anyhit = remaining;
-    ts.use_anyhit = !VOL || S.n_vol == 0;  // without media the first blocking hit decides
+    // Without media the first blocking hit decides.  With one medium a
+    // front-facing hit on any other object (legal under the current
+    // condition) decides too: boundary crossings before it only change the
+    // condition in ways that keep such a hit legal, so walking the crossings
+    // would end at it or earlier.
+    ts.use_anyhit = !VOL || S.n_vol <= 1;
+    if (VOL && S.n_vol == 1) {
+      ts.anyhit_front = true;
+      ts.anyhit_skip = S.vol_obj[0];
+    }
     return true;
   };
   for (;;) {

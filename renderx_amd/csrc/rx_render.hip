@@ -260,6 +260,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     GA.net = G.net();
     GA.Z4 = G.Z4.as<float>();
     GA.X = G.X.as<uint16_t>();
+    RXCK(R.grow.ensure(4 * (size_t)N), "alloc guide rows");
+    GA.row = R.grow.as<uint32_t>();
     GA.rec = R.rec.as<float>();
     GA.R = Rs;
   }
@@ -302,8 +304,10 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       launches++;
       hipEvent_t e1 = timing ? mark() : nullptr;
       if (mode == RX_MODE_GUIDED) {  // network inference for this bounce's hit points
-        hipLaunchKernelGGL(k_guide_query, dim3(blocks), dim3(kRBlock), 0, s, B, GA, b);
-        RXCK(guide_forward(GA.net, GA.X, B.cnt + b, N, nullptr, nullptr, nullptr, c->guide->Z4.as<float>(), c->n_cu, s),
+        if (vol) hipLaunchKernelGGL(k_guide_query<true>, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, GA, b);
+        else hipLaunchKernelGGL(k_guide_query<false>, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, GA, b);
+        RXCK(guide_forward(GA.net, GA.X, B.cnt + kGuideRowBase + b, N, nullptr, nullptr, nullptr,
+                           c->guide->Z4.as<float>(), c->n_cu, s),
              "guide forward");
         if (vol) hipLaunchKernelGGL((k_shade<2, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
         else hipLaunchKernelGGL((k_shade<2, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);

@@ -26,6 +26,7 @@ struct RenderState {
   SceneBufs scene;
   DBuf o, d, beta, lastp, L, s, hit, q0, q1, sq, cnt, so, sd, sc, ss, sum, sumsq, mean, stats;
   DBuf rec, bcount, total;                // guided-mode training records
+  DBuf grow;                              // guided mode: network row of each queue entry
   DBuf bverts, bnv, bvis, bpairs, bsplat;  // bidirectional tracer: subpath vertices, counts, visibility, splats
   uint32_t cap = 0;
   float box_lo[3], box_hi[3];
