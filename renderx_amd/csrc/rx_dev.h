@@ -105,7 +105,8 @@ __host__ __device__ __forceinline__ float slab_tnear(const float lo[3], const fl
 // Certain-miss / certain-hit filter for box_hit.  If box_hit passes a face
 // of axis c at dist t, the real hit point t* = (plane - o_c) / d_c >= 0 lies
 // in every slab a widened by the rounding of the face check, |p_a - p_a*| <=
-// 4u (|t*| |d_a| + B_a) (u = 2^-24, B_a = max |coordinate| of the scene box):
+// 4u (|t*| |d_a| + B_a) (u = 2^-24, B_a = max |coordinate| on axis a of the
+// scene box and the ray origin):
 // in t units e_a - 4u t* - beta_a <= t* <= x_a + 4u t* + beta_a with
 // beta_a = 4u B_a / |d_a| (+ an absolute floor).  The slab ends e_a, x_a are
 // computed here with reciprocals (relative error <= 3u, bounded by 8u with
@@ -122,16 +123,16 @@ constexpr int kBoxUnknown = 0, kBoxMiss = 1, kBoxHit = 2;
 #endif
 __host__ __device__ __forceinline__ int slab_classify(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
                                                       Vec3 beta, float& tnear) {
+  // the set-up (|d_a| >= 2^-30, finite boxes) keeps NaN out of every value
+  // below, so min/max equal the compare-selects of slab_tnear
   const float U8 = 8.0f * 0x1p-24f, U24 = 24.0f * 0x1p-24f;
   const float t0x = (lo[0] - o.x) * inv.x, t1x = (hi[0] - o.x) * inv.x;
   const float t0y = (lo[1] - o.y) * inv.y, t1y = (hi[1] - o.y) * inv.y;
   const float t0z = (lo[2] - o.z) * inv.z, t1z = (hi[2] - o.z) * inv.z;
-  const float ex = t0x < t1x ? t0x : t1x, xx = t0x < t1x ? t1x : t0x;
-  const float ey = t0y < t1y ? t0y : t1y, xy = t0y < t1y ? t1y : t0y;
-  const float ez = t0z < t1z ? t0z : t1z, xz = t0z < t1z ? t1z : t0z;
-  float t = ex;
-  t = ey > t ? ey : t;
-  t = ez > t ? ez : t;
+  const float ex = fminf(t0x, t1x), xx = fmaxf(t0x, t1x);
+  const float ey = fminf(t0y, t1y), xy = fmaxf(t0y, t1y);
+  const float ez = fminf(t0z, t1z), xz = fmaxf(t0z, t1z);
+  const float t = fmaxf(ex, fmaxf(ey, ez));
   tnear = t;
   // widened slab ends: true e_a - beta_a >= le_a, true x_a + beta_a <= hx_a
   const float lx = fmaf(-U8, fabsf(ex), ex) - beta.x, hxx = fmaf(U8, fabsf(xx), xx) + beta.x;
@@ -139,33 +140,25 @@ __host__ __device__ __forceinline__ int slab_classify(const float lo[3], const f
   const float lz = fmaf(-U8, fabsf(ez), ez) - beta.z, hxz = fmaf(U8, fabsf(xz), xz) + beta.z;
   const float L = fmaxf(fmaxf(0.0f, lx), fmaxf(ly, lz));
   const float R = fminf(hxx, fminf(hxy, hxz));
-  if (L * (1.0f - 16.0f * 0x1p-24f) > R * (1.0f + 16.0f * 0x1p-24f)) return kBoxMiss;
+  const bool miss = L * (1.0f - 16.0f * 0x1p-24f) > R * (1.0f + 16.0f * 0x1p-24f);
 #if RX_CERTAIN_HIT == 0
-  return kBoxUnknown;
+  return miss ? kBoxMiss : kBoxUnknown;
 #endif
-  // candidate face: the entry face if the entry is clearly ahead, else the exit face
+  // candidate face: the entry face if the entry is clearly ahead, else the
+  // exit face; its point must lie inside the other two slabs with the
+  // rounding bounds to spare (selects, no branches)
+  const float eux = fmaf(U8, fabsf(ex), ex) + beta.x, xdx = fmaf(-U8, fabsf(xx), xx) - beta.x;
+  const float euy = fmaf(U8, fabsf(ey), ey) + beta.y, xdy = fmaf(-U8, fabsf(xy), xy) - beta.y;
+  const float euz = fmaf(U8, fabsf(ez), ez) + beta.z, xdz = fmaf(-U8, fabsf(xz), xz) - beta.z;
   const bool entry = fmaf(-U8, fabsf(t), t) > 0.0f;
-  float tf = xx;
-  int af = 0;
-  if (entry) {
-    af = (t == ex) ? 0 : ((t == ey) ? 1 : 2);
-    tf = t;
-  } else {
-    tf = xx;
-    if (xy < tf) { tf = xy; af = 1; }
-    if (xz < tf) { tf = xz; af = 2; }
-    if (!(fmaf(-U8, fabsf(tf), tf) > 0.0f)) return kBoxUnknown;
-  }
-  // the face point must lie inside the other two slabs with the rounding bounds to spare
-  const float tlo = tf * (1.0f - U24), thi = tf * (1.0f + U24);
-  const float eu[3] = {fmaf(U8, fabsf(ex), ex) + beta.x, fmaf(U8, fabsf(ey), ey) + beta.y,
-                       fmaf(U8, fabsf(ez), ez) + beta.z};
-  const float xd[3] = {fmaf(-U8, fabsf(xx), xx) - beta.x, fmaf(-U8, fabsf(xy), xy) - beta.y,
-                       fmaf(-U8, fabsf(xz), xz) - beta.z};
-  bool ok = true;
-  for (int a = 0; a < 3; a++)
-    if (a != af) ok = ok && eu[a] <= tlo && thi <= xd[a];
-  return ok ? kBoxHit : kBoxUnknown;
+  const float tx = fminf(xx, fminf(xy, xz));
+  const float tf = entry ? t : tx;
+  // the face's axis: the first axis whose entry (exit) equals tf
+  const bool fx = tf == (entry ? ex : xx), fy = tf == (entry ? ey : xy);
+  const float eo = fx ? fmaxf(euy, euz) : (fy ? fmaxf(eux, euz) : fmaxf(eux, euy));
+  const float xo = fx ? fminf(xdy, xdz) : (fy ? fminf(xdx, xdz) : fminf(xdx, xdy));
+  const bool ok = fmaf(-U8, fabsf(tf), tf) > 0.0f && eo <= tf * (1.0f - U24) && tf * (1.0f + U24) <= xo;
+  return miss ? kBoxMiss : (ok ? kBoxHit : kBoxUnknown);
 }
 // The miss half of slab_classify.
 __host__ __device__ __forceinline__ bool slab_certain_miss(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
@@ -173,14 +166,18 @@ __host__ __device__ __forceinline__ bool slab_certain_miss(const float lo[3], co
   return slab_classify(lo, hi, o, inv, beta, tnear) == kBoxMiss;
 }
 // Per-ray set-up of slab_certain_miss; false when the filter must not be used.
-__host__ __device__ __forceinline__ bool slab_filter_setup(const float root_lo[3], const float root_hi[3], Vec3 d,
-                                                           Vec3 inv, Vec3& beta) {
+// B_a bounds every coordinate the face check adds or subtracts on axis a:
+// the scene box's and the ray origin's (a camera may sit outside the box).
+__host__ __device__ __forceinline__ bool slab_filter_setup(const float root_lo[3], const float root_hi[3], Vec3 o,
+                                                           Vec3 d, Vec3 inv, Vec3& beta) {
   const float m = 0x1p-30f;
   if (!(fabsf(d.x) >= m && fabsf(d.y) >= m && fabsf(d.z) >= m)) return false;
   const float c = 16.0f * 0x1p-24f;
-  beta = v3((c * fmaxf(fabsf(root_lo[0]), fabsf(root_hi[0])) + 0x1p-120f) * fabsf(inv.x) * 1.0001f,
-            (c * fmaxf(fabsf(root_lo[1]), fabsf(root_hi[1])) + 0x1p-120f) * fabsf(inv.y) * 1.0001f,
-            (c * fmaxf(fabsf(root_lo[2]), fabsf(root_hi[2])) + 0x1p-120f) * fabsf(inv.z) * 1.0001f);
+  const float bx = fmaxf(fmaxf(fabsf(root_lo[0]), fabsf(root_hi[0])), fabsf(o.x));
+  const float by = fmaxf(fmaxf(fabsf(root_lo[1]), fabsf(root_hi[1])), fabsf(o.y));
+  const float bz = fmaxf(fmaxf(fabsf(root_lo[2]), fabsf(root_hi[2])), fabsf(o.z));
+  beta = v3((c * bx + 0x1p-120f) * fabsf(inv.x) * 1.0001f, (c * by + 0x1p-120f) * fabsf(inv.y) * 1.0001f,
+            (c * bz + 0x1p-120f) * fabsf(inv.z) * 1.0001f);
   return true;
 }
 
@@ -274,11 +271,15 @@ __device__ __forceinline__ bool trace_begin(const TreeRoot& R, TraceState& s, Ve
   s.bestLeaf = s.bestPos = -1;
   s.sp = 0;
   s.ref = R.ref;
-  if (!box_hit(R.lo, R.hi, o, d)) return false;
   s.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   s.beta = v3(0, 0, 0);
-  s.fast = slab_filter_setup(R.lo, R.hi, d, s.inv, s.beta);
-  return true;
+  s.fast = slab_filter_setup(R.lo, R.hi, o, d, s.inv, s.beta);
+  if (s.fast) {  // the root box as any other: the exact predicate only if undecided
+    float tn;
+    const int c = slab_classify(R.lo, R.hi, o, s.inv, s.beta, tn);
+    if (c != kBoxUnknown) return c == kBoxHit;
+  }
+  return box_hit(R.lo, R.hi, o, d);
 }
 
 // One trip: descend interior nodes to a leaf (or a dead end), test the

@@ -288,6 +288,39 @@ __device__ __forceinline__ uint32_t wave_append_n(uint32_t* counter, uint32_t c)
   base = __shfl(base, 0);
   return base + x - c;
 }
+// Queue appends staged in LDS: the entries a block produces over a chunk of
+// its input go to an LDS list (one LDS atomic per wave), and the block then
+// claims their slots in the global queue with one atomic for the whole chunk.
+// Appends from every wave of the grid otherwise meet at one counter address,
+// where they queue one after another.  The order inside a chunk is the LDS
+// atomics' order, so neighbouring entries stay neighbours.
+constexpr uint32_t kStageIters = 8;                  // input entries per thread per chunk
+constexpr uint32_t kStageCap = kRBlock * kStageIters;  // chunk size and LDS list capacity
+struct StagedQueue {
+  uint32_t* list;   // LDS, kStageCap entries
+  uint32_t* count;  // LDS: [0] entries staged, [1] global base
+};
+__device__ __forceinline__ void staged_reset(StagedQueue q) {
+  if (threadIdx.x == 0) q.count[0] = 0;
+}
+__device__ __forceinline__ void staged_push(StagedQueue q, bool pred, uint32_t v) {
+  const unsigned long long m = __ballot(pred);
+  const int lane = threadIdx.x & 63;
+  const int leader = m ? (__ffsll((long long)m) - 1) : 0;
+  uint32_t base = 0;
+  if (m && lane == leader) base = atomicAdd(q.count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (pred) q.list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+// After a __syncthreads: one thread claims the global slots.
+__device__ __forceinline__ void staged_claim(StagedQueue q, uint32_t* counter) {
+  if (threadIdx.x == 0) q.count[1] = q.count[0] ? atomicAdd(counter, q.count[0]) : 0u;
+}
+// After another __syncthreads: copy the list out.
+__device__ __forceinline__ void staged_flush(StagedQueue q, uint32_t* out) {
+  const uint32_t nq = q.count[0], base = q.count[1];
+  for (uint32_t k = threadIdx.x; k < nq; k += blockDim.x) out[base + k] = q.list[k];
+}
 __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
@@ -723,13 +756,21 @@ template <int MODE, bool BATCH, bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
                                                    BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
+  __shared__ uint32_t cont_list[kStageCap], nee_list[MODE >= 1 ? kStageCap : 1], counts[4];
+  const StagedQueue qc{cont_list, counts}, qn{nee_list, counts + 2};
   const uint32_t n = B.cnt[b];
   const uint32_t* qin = (b & 1) ? B.q1 : B.q0;
   uint32_t* qout = (b & 1) ? B.q0 : B.q1;
-  const uint32_t stride = gridDim.x * blockDim.x;
   uint2* lds = stack + threadIdx.x;
   uint32_t rays = 0, segs = 0;
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+  // block-uniform loops: a chunk of up to kStageCap queue entries per block at
+  // a time, shorter when the queue does not fill the grid that often
+  const uint32_t chunk = blockDim.x * min(kStageIters, max(1u, n / (gridDim.x * blockDim.x)));
+  for (uint32_t c0 = blockIdx.x * chunk; c0 < n; c0 += gridDim.x * chunk) {
+  staged_reset(qc);
+  if (MODE >= 1) staged_reset(qn);
+  __syncthreads();
+  for (uint32_t i0 = c0; i0 < c0 + chunk; i0 += blockDim.x) {
     const uint32_t i = i0 + threadIdx.x;
     bool cont = false, nee = false;
     uint32_t p = 0;
@@ -1005,12 +1046,16 @@ __global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, Pass
         }
       }
     }
-    const uint32_t slot = wave_append(&B.cnt[b + 1], cont);
-    if (cont) qout[slot] = p;
-    if (MODE >= 1) {
-      const uint32_t sslot = wave_append(&B.cnt[kShadowBase + b], nee);
-      if (nee) B.sq[sslot] = p;
-    }
+    staged_push(qc, cont, p);
+    if (MODE >= 1) staged_push(qn, nee, p);
+  }
+  __syncthreads();
+  staged_claim(qc, &B.cnt[b + 1]);
+  if (MODE >= 1) staged_claim(qn, &B.cnt[kShadowBase + b]);
+  __syncthreads();
+  staged_flush(qc, qout);
+  if (MODE >= 1) staged_flush(qn, B.sq);
+  __syncthreads();
   }
   wave_add(&B.stats[0], rays);
   wave_add(&B.stats[3], segs);

@@ -28,7 +28,7 @@ int main(int argc, char** argv) {
     return v3((float)(r * std::cos(p)), (float)(r * std::sin(p)), (float)z);
   };
   long bad = 0;
-  for (int fam = 0; fam < 7; fam++) {
+  for (int fam = 0; fam < 8; fam++) {
     long fast = 0, filtered = 0, exact_miss = 0, exact_hit = 0, certain_hit = 0;
     for (long k = 0; k < n; k++) {
       const double R = std::pow(10.0, (int)(U(rng) * 5) - 1);  // scene half-size 0.1 .. 1000
@@ -65,6 +65,16 @@ int main(int argc, char** argv) {
         }
         d = normalize(v3(p[0], p[1], p[2]) - o);
       }
+      if (fam == 7) {  // origin far outside the scene box (a distant camera), aimed at a face, edge or corner
+        const Vec3 u = unit();
+        const float far = (float)(R * std::pow(10.0, 1 + 3 * U(rng)));
+        o = v3(u.x * far, u.y * far, u.z * far);
+        float p[3];
+        for (int a = 0; a < 3; a++) p[a] = (float)(lo[a] + (hi[a] - lo[a]) * U(rng));
+        const int a = (int)(U(rng) * 3);
+        p[a] = nudge(U(rng) < 0.5 ? lo[a] : hi[a], (int)(U(rng) * 9) - 4);
+        d = normalize(v3(p[0], p[1], p[2]) - o);
+      }
       if (fam == 2) {  // nearly axis-parallel directions
         int a = (int)(U(rng) * 3);
         float s = (float)std::pow(10.0, -7 * U(rng) - 1);
@@ -97,7 +107,7 @@ int main(int argc, char** argv) {
       (exact ? exact_hit : exact_miss)++;
       const Vec3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
       Vec3 beta;
-      if (!slab_filter_setup(rlo, rhi, d, inv, beta)) continue;
+      if (!slab_filter_setup(rlo, rhi, o, d, inv, beta)) continue;
       fast++;
       float tn;
       const int cls = slab_classify(lo, hi, o, inv, beta, tn);

@@ -2,7 +2,8 @@
 never rules out a box that the reference box predicate (src/KDTree.cpp:34-52)
 accepts, and never certifies one it rejects: checked on the CPU over random
 and adversarial rays (aimed at faces, edges and corners with ulp nudges,
-origins on face planes, near axis-parallel directions, flat boxes); its
+origins on face planes, near axis-parallel directions, flat boxes, origins
+far outside the scene box); its
 ordering key equals slab_tnear's."""
 import os
 import subprocess
@@ -23,7 +24,7 @@ def test_certain_miss_filter_is_conservative(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = [l for l in r.stdout.splitlines() if l.startswith("family")]
-    assert len(lines) == 7
+    assert len(lines) == 8
     # the filter must also be useful: random pairs are almost all ruled out
     fam0 = lines[0].split()
     vals = {fam0[i]: int(fam0[i + 1]) for i in range(2, len(fam0) - 1, 2)}
