@@ -33,9 +33,9 @@ namespace rx {
 constexpr int kBptMaxVerts = 32;                  // vertices per subpath (max_depth + 2): one bit each in a mask
 constexpr uint32_t kBptPathSlots = 1u << 20;      // eye + light subpath slots of a pass (default budget)
 constexpr uint32_t kBptMaxPixels = 1u << 22;      // pair codes hold the eye slot in 22 bits
-constexpr int kBptPairCount = kShadowWorkBase + kMaxBounce;  // cnt word: connectable pairs of the pass
-constexpr int kBptVisWork = kBptPairCount + 1;                // cnt word: work counter of the visibility kernel
-static_assert(kBptVisWork < kCntWords, "BPT counters fit the counter block");
+constexpr int kBptPairCount = kSpareWords;  // cnt word: connectable pairs of the pass
+constexpr int kBptVisWork = kSpareWork;     // cnt block: work counters of the visibility kernel
+static_assert(kBptVisWork + (int)kWorkWords <= kCntWords, "BPT counters fit the counter block");
 constexpr double kSplatScale = 4294967296.0;      // fixed point of the light-tracing splats
 
 // One subpath vertex: the fields of the reference's Ray that the connection
@@ -162,13 +162,18 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_generate(DevScene S, DevCam C, 
 // that goes on is queued for the next bounce.
 __global__ void __launch_bounds__(kRBlock) k_bpt_record(DevScene S, PathBufs B, PassArgs P, uint32_t b, BptArgs A) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
+  __shared__ uint32_t cont_list[kStageCap], counts[2];
+  const StagedQueue qc{cont_list, counts};
   const uint32_t n = B.cnt[b];
   const uint32_t* qin = (b & 1) ? B.q1 : B.q0;
   uint32_t* qout = (b & 1) ? B.q0 : B.q1;
-  const uint32_t stride = gridDim.x * blockDim.x;
   uint2* lds = stack + threadIdx.x;
   uint32_t rays = 0, segs = 0;
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+  const uint32_t chunk = blockDim.x * min(kStageIters, max(1u, n / (gridDim.x * blockDim.x)));
+  for (uint32_t c0 = blockIdx.x * chunk; c0 < n; c0 += gridDim.x * chunk) {
+  staged_reset(qc);
+  __syncthreads();
+  for (uint32_t i0 = c0; i0 < c0 + chunk; i0 += blockDim.x) {
     const uint32_t i = i0 + threadIdx.x;
     bool cont = false;
     uint32_t p = 0;
@@ -241,8 +246,13 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_record(DevScene S, PathBufs B, 
         }
       }
     }
-    const uint32_t slot = wave_append(&B.cnt[b + 1], cont);
-    if (cont) qout[slot] = p;
+    staged_push(qc, cont, p);
+  }
+  __syncthreads();
+  staged_claim(qc, &B.cnt[b + 1]);
+  __syncthreads();
+  staged_flush(qc, qout);
+  __syncthreads();
   }
   wave_add(&B.stats[0], rays);
   wave_add(&B.stats[3], segs);
@@ -252,32 +262,64 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_record(DevScene S, PathBufs B, 
 // The (eye vertex j, light vertex k) pairs of every eye slot that
 // colorByConnectingPaths would join (:327-339): neither on an emitter (the
 // light vertex may be the emission point), both sampled RANDOM (connectRays).
-__global__ void __launch_bounds__(kRBlock) k_bpt_pairs(DevScene S, PathBufs B, BptArgs A) {
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t K = A.K, total = A.N * K;
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < total; i0 += stride) {
-    const uint32_t i = i0 + threadIdx.x;
-    uint32_t p = 0, j = 0, mask = 0;
-    if (i < total) {
-      p = i / K;
-      j = i - p * K;
-      if (j < A.nv[p]) {
-        int ce;
-        bool re;
-        vert_flags(A.V, (size_t)p * K + j, ce, re);
-        if (re && !is_emissive(S, ce)) {
-          const uint32_t nL = A.nv[A.N + p];
-          for (uint32_t k = 0; k < nL; k++) {
-            int cl;
-            bool rl;
-            vert_flags(A.V, (size_t)(A.N + p) * K + k, cl, rl);
-            if (rl && !(k >= 1 && is_emissive(S, cl))) mask |= 1u << k;
-          }
-        }
+constexpr uint32_t kPairIters = 8;  // consecutive (slot, eye vertex) entries per thread
+__device__ __forceinline__ uint32_t pair_mask(const DevScene& S, const BptArgs& A, uint32_t p, uint32_t j) {
+  uint32_t mask = 0;
+  if (j < A.nv[p]) {
+    int ce;
+    bool re;
+    vert_flags(A.V, (size_t)p * A.K + j, ce, re);
+    if (re && !is_emissive(S, ce)) {
+      const uint32_t nL = A.nv[A.N + p];
+      for (uint32_t k = 0; k < nL; k++) {
+        int cl;
+        bool rl;
+        vert_flags(A.V, (size_t)(A.N + p) * A.K + k, cl, rl);
+        if (rl && !(k >= 1 && is_emissive(S, cl))) mask |= 1u << k;
       }
     }
-    uint32_t slot = wave_append_n(&B.cnt[kBptPairCount], (uint32_t)__popc(mask));
-    for (; mask; mask &= mask - 1) A.pairs[slot++] = (p << 10) | (j << 5) | (uint32_t)(__ffs(mask) - 1);
+  }
+  return mask;
+}
+__global__ void __launch_bounds__(kRBlock) k_bpt_pairs(DevScene S, PathBufs B, BptArgs A) {
+  __shared__ uint32_t wsum[kRBlock / 64], base;
+  const uint32_t K = A.K, total = A.N * K;
+  const uint32_t chunk = blockDim.x * kPairIters;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (uint32_t c0 = blockIdx.x * chunk; c0 < total; c0 += gridDim.x * chunk) {
+    // thread t: entries c0 + t*kPairIters + [0, kPairIters), i.e. eye vertices of one or two slots
+    const uint32_t e0 = c0 + threadIdx.x * kPairIters;
+    uint32_t masks[kPairIters], cnt = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPairIters; k++) {
+      const uint32_t i = e0 + k;
+      const uint32_t p = i / K;
+      masks[k] = i < total ? pair_mask(S, A, p, i - p * K) : 0u;
+      cnt += (uint32_t)__popc(masks[k]);
+    }
+    // block-wide exclusive prefix of the counts, one atomic per chunk
+    uint32_t x = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      if (w < wid) before += wsum[w];
+      all += wsum[w];
+    }
+    if (threadIdx.x == 0) base = all ? atomicAdd(&B.cnt[kBptPairCount], all) : 0u;
+    __syncthreads();
+    uint32_t slot = base + before + x - cnt;
+#pragma unroll
+    for (uint32_t k = 0; k < kPairIters; k++) {
+      const uint32_t i = e0 + k;
+      const uint32_t p = i / K, j = i - p * K;
+      for (uint32_t m = masks[k]; m; m &= m - 1) A.pairs[slot++] = (p << 10) | (j << 5) | (uint32_t)(__ffs(m) - 1);
+    }
+    __syncthreads();
   }
 }
 
@@ -312,7 +354,7 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_vis(DevScene S, PathBufs B, Bpt
   for (;;) {
     const unsigned long long idle = __ballot(!active);
     if (idle && !ww.drained) {
-      const uint32_t i = wave_take(ww, idle, n, &B.cnt[kBptVisWork]);
+      const uint32_t i = wave_take(ww, idle, n, B.cnt + kBptVisWork);
       if (i < n) {
         code = A.pairs[i];
         const uint32_t p = code >> 10, j = (code >> 5) & 31u, k = code & 31u;

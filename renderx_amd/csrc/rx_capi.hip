@@ -53,7 +53,7 @@ int ensure_spill(rx_ctx* c, int grid_blocks, int block, int depth_needed, int* s
   int sd = std::max(1, depth_needed + 2 - kLdsDepth);
   size_t bytes = (size_t)grid_blocks * block * sd * sizeof(uint2);
   if (int r = hip_err(c, c->spill.ensure(bytes), "spill alloc")) return r;
-  if (int r = hip_err(c, c->overflow.ensure(64), "overflow alloc")) return r;
+  if (int r = hip_err(c, c->overflow.ensure(4 * (1 + kWorkWords)), "overflow alloc")) return r;
   *spill_depth = sd;
   return RX_OK;
 }
@@ -269,7 +269,7 @@ int rx_intersect_dev(rx_ctx* c, uint32_t n, const float* org, const float* dir, 
   a.spill_depth = sd;
   a.overflow = c->overflow.as<int32_t>();
   a.work = c->overflow.as<uint32_t>() + 1;
-  if (int r = hip_err(c, hipMemsetAsync(a.overflow, 0, 8, c->stream), "overflow clear")) return r;
+  if (int r = hip_err(c, hipMemsetAsync(a.overflow, 0, 4 * (1 + kWorkWords), c->stream), "overflow clear")) return r;
   if (int r = hip_err(c, hipEventRecord(c->ev0, c->stream), "event")) return r;
   if (int r = hip_err(c, launch_intersect(c->tree.dev, a, flags, blocks, c->stream), "intersect launch")) return r;
   if (int r = hip_err(c, hipEventRecord(c->ev1, c->stream), "event")) return r;

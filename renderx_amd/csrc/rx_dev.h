@@ -411,18 +411,32 @@ __device__ __forceinline__ Hit trace(const DevTree& T, Vec3 o, Vec3 d, int32_t i
 #endif
 constexpr uint32_t kChunk = RX_CHUNK;  // most queue entries a wave takes from the shared counter at a time
 
+#ifndef RX_WORK_GROUPS
+#define RX_WORK_GROUPS 8
+#endif
+constexpr uint32_t kWorkGroups = RX_WORK_GROUPS;  // counters of one work queue
+constexpr uint32_t kWorkStride = 32;              // words between them: each on its own 128-byte line
+constexpr uint32_t kWorkWords = kWorkGroups * kWorkStride;  // a work queue's counter block (zero at launch)
+
 // A wave's share of a queue [0, n).  Every wave starts on its own chunk of
 // neighbouring entries (wave w: [w*ch, (w+1)*ch)), then takes further chunks
-// from a shared counter (zero at launch) over the rest of the queue, and
-// hands a chunk's entries to its idle lanes.  Neighbouring entries stay on one
-// wave, and the counter keeps every wave busy until the queue is drained,
-// however unevenly the costs fall over the queue.  A chunk is an even share
-// of the queue per wave, capped at kChunk: a short queue is spread thinly over
-// the whole grid, starts all of its rays at once and needs no atomics.
+// from shared counters over the rest of the queue, and hands a chunk's
+// entries to its idle lanes.  Neighbouring entries stay on one wave, and the
+// counters keep every wave busy until the queue is drained, however unevenly
+// the costs fall over the queue.  A chunk is an even share of the queue per
+// wave, capped at kChunk: a short queue is spread thinly over the whole grid,
+// starts all of its rays at once and needs no atomics.  The rest of the queue
+// is cut into kWorkGroups segments with a counter each; a wave starts on the
+// segment of its block's group (blockIdx mod kWorkGroups, the XCD the block
+// runs on when blocks are dealt to XCDs in turn) and moves to the next
+// segment when one is drained, so the atomics spread over kWorkGroups
+// addresses instead of queueing at one.
 struct WaveWork {
   uint32_t next = 0, end = 0;  // the wave's chunk [next, end), wave-uniform
   uint32_t ch = 1;             // chunk size
-  uint32_t dyn = 0;            // start of the counter's part of the queue
+  uint32_t dyn = 0;            // start of the counters' part of the queue
+  uint32_t seg = 0;            // entries per counter segment
+  uint32_t g = 0, left = 0;    // current segment, segments not yet found drained
   bool drained = false;        // nothing left for this wave
 };
 __device__ __forceinline__ WaveWork wave_work(uint32_t n) {
@@ -432,12 +446,15 @@ __device__ __forceinline__ WaveWork wave_work(uint32_t n) {
   w.ch = max(1u, min(kChunk, (n + waves - 1) / waves));
   w.next = min(n, wid * w.ch);
   w.end = min(n, w.next + w.ch);
-  w.dyn = waves * w.ch;
+  w.dyn = min(n, waves * w.ch);
+  w.seg = (n - w.dyn + kWorkGroups - 1) / kWorkGroups;
+  w.g = blockIdx.x % kWorkGroups;
+  w.left = w.dyn < n ? kWorkGroups : 0u;
   return w;
 }
 // Called by every lane of a wave with the wave's ballot of idle lanes
 // (nonzero) while !w.drained; returns the entry for this lane if it is idle,
-// n if there is none.
+// n if there is none.  work: the queue's kWorkWords counter block.
 __device__ __forceinline__ uint32_t wave_take(WaveWork& w, unsigned long long idle, uint32_t n, uint32_t* work) {
   const int lane = threadIdx.x & 63;
   const uint32_t need = (uint32_t)__popcll(idle);
@@ -445,17 +462,33 @@ __device__ __forceinline__ uint32_t wave_take(WaveWork& w, unsigned long long id
   const uint32_t avail = w.end - w.next;
   uint32_t i = w.next + rank;
   if (need > avail) {  // the chunk runs short: the next idle lanes take a new one
-    uint32_t base = n;
-    if (w.dyn < n) {
-      if (lane == 0) base = w.dyn + atomicAdd(work, w.ch);
-      base = __shfl(base, 0);
+    uint32_t base = n, lim = n;
+    if (lane == 0) {
+      while (w.left > 0) {
+        const uint32_t s0 = w.dyn + w.g * w.seg, s1 = min(n, s0 + w.seg);
+        if (s0 < s1) {
+          const uint32_t off = atomicAdd(work + w.g * kWorkStride, w.ch);
+          if (off < s1 - s0) {
+            base = s0 + off;
+            lim = s1;
+            break;
+          }
+        }
+        w.g = (w.g + 1) % kWorkGroups;
+        w.left--;
+      }
     }
-    if (rank >= avail) i = (rank - avail < w.ch) ? base + (rank - avail) : n;
+    base = __shfl(base, 0);
+    lim = __shfl(lim, 0);
+    w.g = __shfl(w.g, 0);
+    w.left = __shfl(w.left, 0);
+    const uint32_t nend = base < n ? min(lim, base + w.ch) : n;
+    if (rank >= avail) i = (base < n && rank - avail < nend - base) ? base + (rank - avail) : n;
     if (base >= n) {
       w.drained = true;
       w.next = w.end = n;
     } else {
-      w.end = min(n, base + w.ch);
+      w.end = nend;
       w.next = min(w.end, base + (need - avail));
     }
   } else {

@@ -33,10 +33,15 @@ constexpr int kExtendMinWaves = RX_EXTEND_MIN_WAVES;
 constexpr int kShadowMinWaves = RX_SHADOW_MIN_WAVES;  // the same for the shadow kernel
 constexpr int kMaxBounce = 64;
 constexpr int kShadowBase = 96;       // cnt[kShadowBase + b]: shadow queue length of bounce b
-constexpr int kWorkBase = 192;       // cnt[kWorkBase + b]: extend work counter of bounce b
-constexpr int kShadowWorkBase = 256; // cnt[kShadowWorkBase + b]: shadow work counter of bounce b
+constexpr int kSpareWords = 192;      // cnt[192, 256): single counters of other passes
 constexpr int kGuideRowBase = 324;   // cnt[kGuideRowBase + b]: network rows of bounce b (guided mode)
-constexpr int kCntWords = kGuideRowBase + kMaxBounce;
+// work counter blocks (kWorkWords each): extend of bounce b at kWorkBase + b * kWorkWords,
+// shadow of bounce b at kShadowWorkBase + b * kWorkWords, one spare block after them
+constexpr int kWorkBase = 512;
+constexpr int kShadowWorkBase = kWorkBase + kMaxBounce * (int)kWorkWords;
+constexpr int kSpareWork = kShadowWorkBase + kMaxBounce * (int)kWorkWords;
+constexpr int kCntWords = kSpareWork + (int)kWorkWords;
+static_assert(kGuideRowBase + kMaxBounce <= kWorkBase, "counter regions do not overlap");
 constexpr uint32_t kDefaultPathSlots = 1u << 25;  // rx_render_tuning.path_slots default (~5.7 GB of path state)
 constexpr uint32_t kGuidedPathSlots = 1u << 22;   // guided mode: the network's per-row buffers scale with the slots
 constexpr int kMaxCross = 4;          // index-matched boundary crossings per shadow ray
@@ -427,7 +432,7 @@ __global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend(DevTree T, 
   uint32_t boxes = 0, tris = 0;
   bool ovf = false;
   trace_range<false, LEG, STATS, kLdsDepth>(
-      T, n, &B.cnt[kWorkBase + b], stack + threadIdx.x, B.spill + (size_t)gtid * B.spill_depth, B.spill_depth, &ovf,
+      T, n, B.cnt + kWorkBase + b * kWorkWords, stack + threadIdx.x, B.spill + (size_t)gtid * B.spill_depth, B.spill_depth, &ovf,
       [&](uint32_t i, Vec3& o, Vec3& d, int32_t& ins, int32_t& con, TreeRoot& root) {
         const uint32_t p = q[i];
         o = xyz(B.o[p]);
@@ -502,36 +507,57 @@ __device__ __forceinline__ bool guided_vertex(const DevScene& S, int inside, int
 // compacted into rows [0, cnt[kGuideRowBase + b]); row[i] maps queue entry i
 // to its row.
 template <bool VOL>
+__device__ __forceinline__ bool guide_input(const DevScene& S, const PathBufs& B, const PassArgs& P, uint32_t b,
+                                            uint32_t p, Vec3& x, Vec3& dd) {
+  uint32_t pix, smp;
+  slot_pixel(P, p, pix, smp);
+  const float4 o = B.o[p], d = B.d[p];
+  const int2 h = B.hit[p];
+  const int pos = h.y;
+  const float t = __int_as_float(h.x);
+  int hitObj = NONE;
+  Vec3 hn = v3(0, 0, 0);
+  if (pos >= 0) {
+    hitObj = __float_as_int(S.tree.tv1[pos].w);
+    hn = xyz(S.tree.tn[pos]);
+  }
+  dd = xyz(d);
+  x = xyz(o) + dd * (pos >= 0 ? t : 0.0f);
+  const float u5 = VOL ? u01(philox(U4{smp, b, 1u, 0u}, pix, P.key1).y) : 0.0f;
+  return guided_vertex<VOL>(S, B.s[p].x, hitObj, pos, t, dd, hn, u5);
+}
+template <bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_guide_query(DevScene S, PathBufs B, PassArgs P, GuideArgs GA, uint32_t b) {
+  __shared__ uint32_t list[kStageCap], counts[2];
+  const StagedQueue qg{list, counts};
   const uint32_t n = B.cnt[b];
   const uint32_t* q = (b & 1) ? B.q1 : B.q0;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
-    const uint32_t i = i0 + threadIdx.x;
-    bool want = false;
-    Vec3 x = v3(0, 0, 0), dd = v3(0, 0, 0);
-    if (i < n) {
-      const uint32_t p = q[i];
-      uint32_t pix, smp;
-      slot_pixel(P, p, pix, smp);
-      const float4 o = B.o[p], d = B.d[p];
-      const int2 h = B.hit[p];
-      const int pos = h.y;
-      const float t = __int_as_float(h.x);
-      int hitObj = NONE;
-      Vec3 hn = v3(0, 0, 0);
-      if (pos >= 0) {
-        hitObj = __float_as_int(S.tree.tv1[pos].w);
-        hn = xyz(S.tree.tn[pos]);
+  const uint32_t chunk = blockDim.x * min(kStageIters, max(1u, n / (gridDim.x * blockDim.x)));
+  for (uint32_t c0 = blockIdx.x * chunk; c0 < n; c0 += gridDim.x * chunk) {
+    staged_reset(qg);
+    __syncthreads();
+    for (uint32_t i0 = c0; i0 < c0 + chunk; i0 += blockDim.x) {
+      const uint32_t i = i0 + threadIdx.x;
+      bool want = false;
+      if (i < n) {
+        Vec3 x, dd;
+        want = guide_input<VOL>(S, B, P, b, q[i], x, dd);
+        if (!want) GA.row[i] = kNoRow;
       }
-      dd = xyz(d);
-      const float u5 = VOL ? u01(philox(U4{smp, b, 1u, 0u}, pix, P.key1).y) : 0.0f;
-      want = guided_vertex<VOL>(S, B.s[p].x, hitObj, pos, t, dd, hn, u5);
-      x = xyz(o) + dd * (pos >= 0 ? t : 0.0f);
+      staged_push(qg, want, i);
     }
-    const uint32_t r = wave_append(&B.cnt[kGuideRowBase + b], want);
-    if (i < n) GA.row[i] = want ? r : kNoRow;
-    if (want) guide_encode(GA.net, x, -dd, GA.X + (size_t)r * kGIn);
+    __syncthreads();
+    staged_claim(qg, &B.cnt[kGuideRowBase + b]);
+    __syncthreads();
+    // the staged entries get rows base + k; their inputs are read again (cached)
+    for (uint32_t k = threadIdx.x; k < counts[0]; k += blockDim.x) {
+      const uint32_t i = list[k], r = counts[1] + k;
+      Vec3 x, dd;
+      guide_input<VOL>(S, B, P, b, q[i], x, dd);
+      GA.row[i] = r;
+      guide_encode(GA.net, x, -dd, GA.X + (size_t)r * kGIn);
+    }
+    __syncthreads();
   }
 }
 
@@ -1129,7 +1155,7 @@ __global__ void __launch_bounds__(kRBlock, kShadowMinWaves) k_shadow(DevScene S0
   for (;;) {
     const unsigned long long idle = __ballot(!active);
     if (idle && !ww.drained) {
-      const uint32_t i = wave_take(ww, idle, n, &B.cnt[kShadowWorkBase + b]);
+      const uint32_t i = wave_take(ww, idle, n, B.cnt + kShadowWorkBase + b * kWorkWords);
       {
         if (i < n) {
           p = B.sq[i];
