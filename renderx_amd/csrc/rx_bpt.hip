@@ -476,15 +476,6 @@ __device__ __forceinline__ float self_pdf(const BVert& r) {
   return lobe_pdf(r.n, r.d, 1.0f);
 }
 
-// BidirectionalPathTracer::link (:183-195)
-__device__ __forceinline__ BVert bpt_link(const BView& v, int ci, int i, int j) {
-  BVert src = v.get(min(max(i, 0), v.W - 1));
-  if ((j > i && i <= ci) || (j < i && i > ci)) return src;
-  const BVert dst = v.get(min(max(j, 0), v.W - 1));
-  src.d = nrm0(dst.o - src.o);
-  src.inside = dst.inside;
-  return src;
-}
 __device__ __forceinline__ bool must_use_pt(const BView& v) {
   for (int i = 0; i + 1 < v.W; i++)
     if (v.random(i) && v.random(i + 1)) return false;
@@ -513,46 +504,60 @@ __device__ void connect_color_prob(const DevScene& S, const BView& v, int ci, Ve
     cur = next;
   }
 }
+// BidirectionalPathTracer::link (:183-195): vertex i (src) as seen from the
+// connected path, re-aimed at vertex j (dst) where the path runs against its
+// own sampling direction
+__device__ __forceinline__ BVert link_to(BVert src, const BVert& dst, int ci, int i, int j) {
+  if ((j > i && i <= ci) || (j < i && i > ci)) return src;
+  src.d = nrm0(dst.o - src.o);
+  src.inside = dst.inside;
+  return src;
+}
 // connectWeight (:197-263) with expTerm = 1, in double as the reference; the
-// backward densities first, then the forward ones accumulated on the fly
+// backward densities first, then the forward ones accumulated on the fly.
+// Each loop slides a window of three neighbouring vertices along the path,
+// so every vertex is read once per loop; the forward loop also finds whether
+// any two neighbours are both RANDOM (mustUsePT otherwise).
 __device__ float connect_weight(const DevScene& S, const BView& v, int ci) {
   const int W = v.W;
   double pb[2 * kBptMaxVerts];
-  pb[W - 1] = v.get(W - 1).originProb;
+  BVert a{}, b = v.get(W - 1), c;  // vertices i + 1, i, i - 1
+  pb[W - 1] = b.originProb;
   for (int i = W - 1; i > 0; i--) {
-    const float dl = fmaxf(length(v.get(i).o - v.get(i - 1).o), 1e-5f);
-    pb[i - 1] = pb[i] * (double)bpt_cos(bpt_link(v, ci, i - 1, i));
+    c = v.get(i - 1);
+    const float dl = fmaxf(length(b.o - c.o), 1e-5f);
+    pb[i - 1] = pb[i] * (double)bpt_cos(link_to(c, b, ci, i - 1, i));
     pb[i - 1] /= (double)dl * (double)dl;
-    if (v.random(i)) {
-      if (i < W - 1) {
-        const BVert in = bpt_link(v, ci, i + 1, i);
-        pb[i - 1] *= (double)bpt_pdf(S, in.d, bpt_link(v, ci, i, i - 1));
-      } else {
-        pb[i - 1] *= (double)self_pdf(bpt_link(v, ci, i, i - 1));
-      }
+    if (b.random) {
+      if (i < W - 1) pb[i - 1] *= (double)bpt_pdf(S, link_to(a, b, ci, i + 1, i).d, link_to(b, c, ci, i, i - 1));
+      else pb[i - 1] *= (double)self_pdf(link_to(b, c, ci, i, i - 1));
     }
+    a = b;
+    b = c;
   }
-  double pf = v.get(0).originProb, sum = 0.0, selfF = 0.0;
-  bool prev_random = v.random(0);
+  a = BVert{};
+  b = v.get(0);  // vertices i - 1, i, i + 1
+  double pf = b.originProb, sum = 0.0, selfF = 0.0;
+  bool any_pair = false;
   for (int i = 0; i < W - 1; i++) {
     if (i == ci) selfF = pf;
-    const bool next_random = v.random(i + 1);
-    if (prev_random && next_random) sum += pf * pb[i + 1];
-    const float dl = fmaxf(length(v.get(i + 1).o - v.get(i).o), 1e-5f);
-    double nf = pf * (double)bpt_cos(bpt_link(v, ci, i + 1, i));
+    c = v.get(i + 1);
+    if (b.random && c.random) {
+      sum += pf * pb[i + 1];
+      any_pair = true;
+    }
+    const float dl = fmaxf(length(c.o - b.o), 1e-5f);
+    double nf = pf * (double)bpt_cos(link_to(c, b, ci, i + 1, i));
     nf /= (double)dl * (double)dl;
-    if (prev_random) {
-      if (i > 0) {
-        const BVert in = bpt_link(v, ci, i - 1, i);
-        nf *= (double)bpt_pdf(S, in.d, bpt_link(v, ci, i, i + 1));
-      } else {
-        nf *= (double)self_pdf(bpt_link(v, ci, i, i + 1));
-      }
+    if (b.random) {
+      if (i > 0) nf *= (double)bpt_pdf(S, link_to(a, b, ci, i - 1, i).d, link_to(b, c, ci, i, i + 1));
+      else nf *= (double)self_pdf(link_to(b, c, ci, i, i + 1));
     }
     pf = nf;
-    prev_random = next_random;
+    a = b;
+    b = c;
   }
-  if (must_use_pt(v)) sum += pb[0];
+  if (!any_pair) sum += pb[0];  // mustUsePT
   const double self = ci == -1 ? pb[0] : selfF * pb[ci + 1];
   const double res = self / sum;
   if (!(res >= 0)) return 0.0f;

@@ -478,10 +478,11 @@ __device__ __forceinline__ uint32_t wave_take(WaveWork& w, unsigned long long id
         w.left--;
       }
     }
-    base = __shfl(base, 0);
-    lim = __shfl(lim, 0);
-    w.g = __shfl(w.g, 0);
-    w.left = __shfl(w.left, 0);
+    // lane 0's values, as wave-uniform (scalar) values
+    base = __builtin_amdgcn_readfirstlane(base);
+    lim = __builtin_amdgcn_readfirstlane(lim);
+    w.g = __builtin_amdgcn_readfirstlane(w.g);
+    w.left = __builtin_amdgcn_readfirstlane(w.left);
     const uint32_t nend = base < n ? min(lim, base + w.ch) : n;
     if (rank >= avail) i = (base < n && rank - avail < nend - base) ? base + (rank - avail) : n;
     if (base >= n) {

@@ -23,7 +23,7 @@ constexpr int NONE = -1, CAMERA = -2;
 static_assert(kNoCondition != NONE && kNoCondition != CAMERA, "distinct contact sentinels");
 constexpr int kRBlock = kTraceBlock;  // render kernels share the traversal block shape
 #ifndef RX_EXTEND_MIN_WAVES
-#define RX_EXTEND_MIN_WAVES 7
+#define RX_EXTEND_MIN_WAVES 6
 #endif
 // minimum resident waves per SIMD the extend kernel's registers must allow
 constexpr int kExtendMinWaves = RX_EXTEND_MIN_WAVES;
