@@ -79,6 +79,7 @@ struct BptArgs {
   uint32_t* vis;     // N x K masks: bit k of (p, j) = eye vertex j of slot p sees light vertex k
   uint32_t* pairs;   // connectable pairs: p << 10 | j << 5 | k
   unsigned long long* splat;  // S x npix x 3 fixed-point light-tracing sums
+  unsigned long long* own;    // N x 3 fixed-point sums of each eye slot's connected paths
   uint32_t N, K;
   uint32_t registered;  // RX_RENDER_BPT_REGISTERED
 };
@@ -564,119 +565,142 @@ __device__ float connect_weight(const DevScene& S, const BView& v, int ci) {
   return (float)res;
 }
 
-// colorByConnectingPaths for eye slot p: the eye path's own pixel collects
-// the contributions with two or more eye vertices (into L[p]); those with the
-// camera vertex alone go to the pixel Camera::transToPixel gives the light
-// vertex (src/Camera.cpp:136-152), scaled by the light-tracing factors
-// (:357-373), as fixed point into the pass's splat image of sample j.
-__global__ void __launch_bounds__(kRBlock) k_bpt_connect(DevScene S, DevCam C, PathBufs B, PassArgs P, BptArgs A) {
+// One split of colorByConnectingPaths for eye slot p: the W-vertex path of
+// light prefix lp and eye prefix W - lp, after connectRays
+// (src/MCRenderer.cpp:50-106); false when it contributes nothing, else its
+// color / prob * weight.  Splits with lp >= 1 are the pair list's, already
+// filtered as the reference's loop filters them (:327-339); the visibility
+// bit is k_bpt_vis's.
+__device__ bool bpt_split(const DevScene& S, const BptArgs& A, uint32_t p, int W, int lp, BView& v, Vec3& out) {
+  const int ep = W - lp, ci = lp - 1;
+  v.V = A.V;
+  v.eb = (size_t)p * A.K;
+  v.lb = (size_t)(A.N + p) * A.K;
+  v.W = W;
+  v.lp = lp;
+  if (lp == 0) {
+    int cf;
+    bool rf;
+    vert_flags(A.V, v.eb + (W - 1), cf, rf);
+    if (!is_emissive(S, cf)) return false;
+    if (!must_use_pt(v)) return false;
+  } else {
+    if (!((A.vis[(size_t)p * A.K + ep - 1] >> (lp - 1)) & 1u)) return false;
+    BVert lv = get_vert(A.V, v.lb + lp - 1), ev = get_vert(A.V, v.eb + ep - 1);
+    // a missing previous ray is the ray itself
+    const BVert pl = lp >= 2 ? get_vert(A.V, v.lb + lp - 2) : lv;
+    const BVert pe = ep >= 2 ? get_vert(A.V, v.eb + ep - 2) : ev;
+    lv.d = nrm0(ev.o - lv.o);
+    ev.d = -lv.d;
+    lv.dirProb = 1.0f;
+    lv.c = bpt_bsdf(S, lp >= 2 ? pl.d : lv.d, lv);
+    lv.random = true;
+    if (lv.contact != NONE && lp >= 2) {
+      if (dot(lv.n, lv.d) <= 0) lv.inside = lv.contact;
+      else if (dot(lv.n, pl.d) <= 0) lv.inside = pl.inside;
+      else lv.inside = NONE;
+    }
+    ev.dirProb = 1.0f;
+    ev.c = bpt_bsdf(S, ep >= 2 ? pe.d : ev.d, ev);
+    ev.random = true;
+    eye_connect_state(ev, pe, ep >= 2, ev.d, ev.inside);
+    v.lc = lv;
+    v.ec = ev;
+  }
+  Vec3 color;
+  float prob;
+  connect_color_prob(S, v, ci, color, prob);
+  if (!(prob > 0)) return false;
+  if (!(length(color) > 0)) return false;
+  const float weight = connect_weight(S, v, ci);
+  out = v3(color.x / prob, color.y / prob, color.z / prob) * weight;
+  return true;
+}
+
+// 2^-32 fixed point into a 64-bit integer sum: additions in any order give
+// the same bits.
+__device__ __forceinline__ void add_fixed(unsigned long long* dst, Vec3 c) {
+  const float cc[3] = {c.x, c.y, c.z};
+  for (int ch = 0; ch < 3; ch++) {
+    const double fx = (double)cc[ch] * kSplatScale + 0.5;
+    if (fx >= 1.0 && fx < 9.0e18) atomicAdd(dst + ch, (unsigned long long)fx);
+  }
+}
+
+// Paths that are the eye subpath alone, ending on an emitter (lp = 0): one
+// thread per eye slot, into L[p].
+__global__ void __launch_bounds__(kRBlock) k_bpt_connect(DevScene S, PathBufs B, BptArgs A) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < A.N; p += stride) {
+    const int nE = (int)A.nv[p];
+    Vec3 own = v3(0, 0, 0);
+    BView v;
+    for (int W = 2; W <= nE; W++) {
+      Vec3 c;
+      if (bpt_split(S, A, p, W, 0, v, c)) own = own + c;
+    }
+    B.L[p] = make_float4(own.x, own.y, own.z, 0.0f);
+  }
+}
+
+// colorByConnectingPaths' connected splits (light prefix lp >= 1), one
+// thread per entry of the pair list: paths with two or more eye vertices add
+// to their eye slot's own fixed-point sum; those with the camera vertex alone
+// go to the pixel Camera::transToPixel gives the light vertex
+// (src/Camera.cpp:136-152), scaled by the light-tracing factors (:357-373),
+// into the pass's splat image of the slot's sample.
+__global__ void __launch_bounds__(kRBlock) k_bpt_connect_pairs(DevScene S, DevCam C, PathBufs B, PassArgs P,
+                                                               BptArgs A) {
+  const uint32_t n = B.cnt[kBptPairCount];
   const uint32_t stride = gridDim.x * blockDim.x;
   const Vec3 front = ld3(C.front), right = ld3(C.right), up = ld3(C.up), cpos = ld3(C.pos);
   const uint32_t npix = P.npix;
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < A.N; p += stride) {
-    const int nE = (int)A.nv[p], nL = (int)A.nv[A.N + p];
-    const uint32_t vis_base = p * A.K;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t code = A.pairs[i];
+    const uint32_t p = code >> 10, j = (code >> 5) & 31u, k = code & 31u;
+    const int ep = (int)j + 1, lp = (int)k + 1;
+    BView v;
+    Vec3 color;
+    if (!bpt_split(S, A, p, ep + lp, lp, v, color)) continue;
+    if (ep >= 2) {
+      add_fixed(A.own + 3 * (size_t)p, color);
+      continue;
+    }
     uint32_t jpix, jsmp;
     slot_pixel(P, p, jpix, jsmp);
     jsmp -= P.sample;  // the sample's index within the pass
-    Vec3 own = v3(0, 0, 0);
-    BView v;
-    v.V = A.V;
-    v.eb = (size_t)p * A.K;
-    v.lb = (size_t)(A.N + p) * A.K;
-    for (int W = 2; W <= nE + nL; W++) {
-      const int lps = W - nE > 0 ? W - nE : 0;
-      const int lpe = W - 1 < nL ? W - 1 : nL;
-      for (int lp = lps; lp <= lpe; lp++) {
-        const int ep = W - lp;
-        v.W = W;
-        v.lp = lp;
-        const int ci = lp - 1;
-        if (lp == 0) {
-          int cf;
-          bool rf;
-          vert_flags(A.V, v.eb + (W - 1), cf, rf);
-          if (!is_emissive(S, cf)) continue;
-          if (!must_use_pt(v)) continue;
-        } else {
-          BVert lv = get_vert(A.V, v.lb + lp - 1), ev = get_vert(A.V, v.eb + ep - 1);
-          if (lp > 1 && is_emissive(S, lv.contact)) continue;
-          if (is_emissive(S, ev.contact)) continue;
-          if (!lv.random || !ev.random) continue;
-          if (!((A.vis[vis_base + ep - 1] >> (lp - 1)) & 1u)) continue;
-          // connectRays (src/MCRenderer.cpp:50-106); a missing previous ray is the ray itself
-          const BVert pl = lp >= 2 ? get_vert(A.V, v.lb + lp - 2) : lv;
-          const BVert pe = ep >= 2 ? get_vert(A.V, v.eb + ep - 2) : ev;
-          lv.d = nrm0(ev.o - lv.o);
-          ev.d = -lv.d;
-          lv.dirProb = 1.0f;
-          lv.c = bpt_bsdf(S, lp >= 2 ? pl.d : lv.d, lv);
-          lv.random = true;
-          if (lv.contact != NONE && lp >= 2) {
-            if (dot(lv.n, lv.d) <= 0) lv.inside = lv.contact;
-            else if (dot(lv.n, pl.d) <= 0) lv.inside = pl.inside;
-            else lv.inside = NONE;
-          }
-          ev.dirProb = 1.0f;
-          ev.c = bpt_bsdf(S, ep >= 2 ? pe.d : ev.d, ev);
-          ev.random = true;
-          eye_connect_state(ev, pe, ep >= 2, ev.d, ev.inside);
-          v.lc = lv;
-          v.ec = ev;
-        }
-        Vec3 color;
-        float prob;
-        connect_color_prob(S, v, ci, color, prob);
-        if (!(prob > 0)) continue;
-        if (!(length(color) > 0)) continue;
-        const float weight = connect_weight(S, v, ci);
-        color = v3(color.x / prob, color.y / prob, color.z / prob) * weight;
-        if (ep == 1) {
-          const Vec3 lo = v.lc.o;
-          const Vec3 dir = nrm0(lo - v.ec.o);
-          const float cosAtCamera = fabsf(dot(front, dir));
-          const uint32_t sd = (uint32_t)C.sight;
-          const float imageToSolidAngleFactor = (float)(sd * sd) / (cosAtCamera * cosAtCamera * cosAtCamera);
-          const float imageToSurfaceFactor = imageToSolidAngleFactor / cosAtCamera;
-          color = color * (imageToSurfaceFactor / (float)npix);
-          color = color * cosAtCamera;
-          const Vec3 q = lo - cpos;
-          const float ratio = C.sight / dot(q, front);
-          const float px = dot(q, right) * ratio + ((float)C.W / 2.0f - 0.5f);
-          const float py = (float)C.H / 2.0f - dot(q, up) * ratio - 0.5f;
-          int ix = -1, iy = -1;
-          if (A.registered) {  // the pixel whose camera rays cover the raster position
-            const float fx = floorf(px + 0.5f), fy = floorf(py + 0.5f);
-            if (fx >= 0.0f && fx < (float)C.W && fy >= 0.0f && fy < (float)C.H) {
-              ix = (int)fx;
-              iy = (int)fy;
-            }
-          } else if (px > -1.0f && px < (float)C.W && py > -1.0f && py < (float)C.H) {  // (int) truncates toward 0
-            ix = (int)px;
-            iy = (int)py;
-          }
-          if (ix >= 0) {
-            const uint32_t pixel = (uint32_t)(iy * C.W + ix);
-            unsigned long long* dst = A.splat + 3 * ((size_t)jsmp * npix + pixel);
-            const float cc[3] = {color.x, color.y, color.z};
-            for (int ch = 0; ch < 3; ch++) {
-              const double fx = (double)cc[ch] * kSplatScale + 0.5;
-              if (fx >= 1.0 && fx < 9.0e18) atomicAdd(dst + ch, (unsigned long long)fx);
-            }
-          }
-        } else {
-          own = own + color;
-        }
+    const Vec3 lo = v.lc.o;
+    const Vec3 dir = nrm0(lo - v.ec.o);
+    const float cosAtCamera = fabsf(dot(front, dir));
+    const uint32_t sd = (uint32_t)C.sight;
+    const float imageToSolidAngleFactor = (float)(sd * sd) / (cosAtCamera * cosAtCamera * cosAtCamera);
+    const float imageToSurfaceFactor = imageToSolidAngleFactor / cosAtCamera;
+    color = color * (imageToSurfaceFactor / (float)npix);
+    color = color * cosAtCamera;
+    const Vec3 q = lo - cpos;
+    const float ratio = C.sight / dot(q, front);
+    const float px = dot(q, right) * ratio + ((float)C.W / 2.0f - 0.5f);
+    const float py = (float)C.H / 2.0f - dot(q, up) * ratio - 0.5f;
+    int ix = -1, iy = -1;
+    if (A.registered) {  // the pixel whose camera rays cover the raster position
+      const float fx = floorf(px + 0.5f), fy = floorf(py + 0.5f);
+      if (fx >= 0.0f && fx < (float)C.W && fy >= 0.0f && fy < (float)C.H) {
+        ix = (int)fx;
+        iy = (int)fy;
       }
+    } else if (px > -1.0f && px < (float)C.W && py > -1.0f && py < (float)C.H) {  // (int) truncates toward 0
+      ix = (int)px;
+      iy = (int)py;
     }
-    B.L[p] = make_float4(own.x, own.y, own.z, 0.0f);
+    if (ix >= 0) add_fixed(A.splat + 3 * ((size_t)jsmp * npix + (uint32_t)(iy * C.W + ix)), color);
   }
 }
 
 // Iteration images in sample order: own + splatted radiance of sample j,
 // divided by the vignetting factor (MCRenderer::eliminateVignetting), added
 // to the FP64 sums.
-__global__ void k_bpt_accumulate(PathBufs B, PassArgs P, DevCam C, const unsigned long long* splat) {
+__global__ void k_bpt_accumulate(PathBufs B, PassArgs P, DevCam C, const unsigned long long* splat,
+                                 const unsigned long long* ownfix) {
   const uint32_t stride = gridDim.x * blockDim.x;
   const Vec3 front = ld3(C.front), right = ld3(C.right), up = ld3(C.up);
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < P.npix; q += stride) {
@@ -690,11 +714,13 @@ __global__ void k_bpt_accumulate(PathBufs B, PassArgs P, DevCam C, const unsigne
       s2[k] = B.sumsq[3 * (size_t)q + k];
     }
     for (uint32_t j = 0; j < P.spass; j++) {
-      const float4 L = B.L[pass_slot(P, q, j)];
+      const uint32_t slot = pass_slot(P, q, j);
+      const float4 L = B.L[slot];
       const unsigned long long* sp = splat + 3 * ((size_t)j * P.npix + q);
+      const unsigned long long* ow = ownfix + 3 * (size_t)slot;
       const float own[3] = {L.x, L.y, L.z};
       for (int k = 0; k < 3; k++) {
-        const float val = (own[k] + (float)((double)sp[k] / kSplatScale)) / vig;
+        const float val = (own[k] + (float)((double)ow[k] / kSplatScale) + (float)((double)sp[k] / kSplatScale)) / vig;
         s[k] += val;
         s2[k] += (double)val * val;
       }
@@ -751,6 +777,7 @@ int render_bpt(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm
   RXCK(R.bvis.ensure(4 * (size_t)N * K), "alloc");
   RXCK(R.bpairs.ensure(4 * (size_t)N * K * K), "alloc pairs");
   RXCK(R.bsplat.ensure(24 * (size_t)N), "alloc splats");
+  RXCK(R.bown.ensure(24 * (size_t)N), "alloc own sums");
   RXCK(R.sum.ensure(24 * (size_t)npix), "alloc");
   RXCK(R.sumsq.ensure(24 * (size_t)npix), "alloc");
   RXCK(R.mean.ensure(12 * (size_t)npix), "alloc");
@@ -795,6 +822,7 @@ int render_bpt(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm
   A.vis = R.bvis.as<uint32_t>();
   A.pairs = R.bpairs.as<uint32_t>();
   A.splat = R.bsplat.as<unsigned long long>();
+  A.own = R.bown.as<unsigned long long>();
   A.N = N;
   A.K = K;
   A.registered = (prm->flags & RX_RENDER_BPT_REGISTERED) ? 1u : 0u;
@@ -824,6 +852,7 @@ int render_bpt(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm
     done += spass;
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, A.vis, (size_t)A.N * K);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, reinterpret_cast<double*>(A.splat), 3 * (size_t)A.N);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, reinterpret_cast<double*>(A.own), 3 * (size_t)A.N);
     hipLaunchKernelGGL(k_bpt_generate, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, A);
     for (uint32_t b = 0; b <= prm->max_depth; b++) {
       hipLaunchKernelGGL((k_extend<true, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, b, BA0);
@@ -832,9 +861,10 @@ int render_bpt(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm
     }
     hipLaunchKernelGGL(k_bpt_pairs, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
     hipLaunchKernelGGL(k_bpt_vis, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
-    hipLaunchKernelGGL(k_bpt_connect, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, A);
+    hipLaunchKernelGGL(k_bpt_connect, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
+    hipLaunchKernelGGL(k_bpt_connect_pairs, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, A);
     hipLaunchKernelGGL(k_bpt_accumulate, dim3(blocks), dim3(kRBlock), 0, s, B, P, C,
-                       (const unsigned long long*)A.splat);
+                       (const unsigned long long*)A.splat, (const unsigned long long*)A.own);
     RXCK(hipGetLastError(), "launch pass");
   }
   RXCK(hipEventRecord(c->ev1, s), "event");

@@ -27,7 +27,7 @@ struct RenderState {
   DBuf o, d, beta, lastp, L, s, hit, q0, q1, sq, cnt, so, sd, sc, ss, sum, sumsq, mean, stats;
   DBuf rec, bcount, total;                // guided-mode training records
   DBuf grow;                              // guided mode: network row of each queue entry
-  DBuf bverts, bnv, bvis, bpairs, bsplat;  // bidirectional tracer: subpath vertices, counts, visibility, splats
+  DBuf bverts, bnv, bvis, bpairs, bsplat, bown;  // bidirectional tracer: subpath vertices, counts, visibility, splats, own sums
   uint32_t cap = 0;
   float box_lo[3], box_hi[3];
 };
