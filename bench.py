@@ -193,8 +193,8 @@ def bpt_cornell(res=256, spp=64, cpu=True):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
