@@ -28,7 +28,7 @@ constexpr int kRBlock = kTraceBlock;  // render kernels share the traversal bloc
 // minimum resident waves per SIMD the extend kernel's registers must allow
 constexpr int kExtendMinWaves = RX_EXTEND_MIN_WAVES;
 #ifndef RX_SHADOW_MIN_WAVES
-#define RX_SHADOW_MIN_WAVES 1
+#define RX_SHADOW_MIN_WAVES 4
 #endif
 constexpr int kShadowMinWaves = RX_SHADOW_MIN_WAVES;  // the same for the shadow kernel
 constexpr int kMaxBounce = 64;
