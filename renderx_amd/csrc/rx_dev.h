@@ -409,7 +409,7 @@ __device__ __forceinline__ Hit trace(const DevTree& T, Vec3 o, Vec3 d, int32_t i
 #ifndef RX_CHUNK
 #define RX_CHUNK 64
 #endif
-constexpr uint32_t kChunk = RX_CHUNK;  // most queue entries a wave takes from the shared counter at a time
+constexpr uint32_t kChunk = RX_CHUNK;  // most queue entries a wave takes from a work counter at a time
 
 #ifndef RX_WORK_GROUPS
 #define RX_WORK_GROUPS 8

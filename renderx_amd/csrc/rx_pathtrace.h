@@ -269,30 +269,6 @@ __device__ __forceinline__ bool refract(Vec3 d, Vec3 n, float n_i, float n_t, Ve
   return true;
 }
 
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-  const unsigned long long m = __ballot(pred);
-  const int lane = threadIdx.x & 63;
-  const int leader = m ? (__ffsll((long long)m) - 1) : 0;
-  uint32_t base = 0;
-  if (m && lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-// Appends c entries per lane with one atomic per wave; returns the lane's
-// first slot (lanes in order).
-__device__ __forceinline__ uint32_t wave_append_n(uint32_t* counter, uint32_t c) {
-  const int lane = threadIdx.x & 63;
-  uint32_t x = c;  // inclusive prefix sum over the wave
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off);
-    if (lane >= off) x += y;
-  }
-  const uint32_t total = __shfl(x, 63);
-  uint32_t base = 0;
-  if (lane == 0 && total) base = atomicAdd(counter, total);
-  base = __shfl(base, 0);
-  return base + x - c;
-}
 // Queue appends staged in LDS: the entries a block produces over a chunk of
 // its input go to an LDS list (one LDS atomic per wave), and the block then
 // claims their slots in the global queue with one atomic for the whole chunk.

@@ -2,7 +2,7 @@
 // (Shader/Intersection.frag:25-84, one fragment per ray) with the CPU
 // semantics of KDTree::intersect (src/KDTree.cpp:295-339).
 //
-// Each wave takes chunks of the batch from a shared counter, one ray per
+// Each wave takes chunks of the batch from shared counters, one ray per
 // lane, and lanes take the chunk's next ray as theirs ends (trace_range); the
 // traversal stack is a per-lane LDS column (conflict-free: lane-contiguous)
 // with a per-thread global spill for deep trees.
