@@ -109,6 +109,30 @@ def test_empty_and_edge_batches(rx, gpu_ctx):
     assert g["tid"][0] == -1 and g["dist"][0] == -1
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4095, 65537, 458751, 458753, 917505, 1000003])
+def test_work_distribution_covers_every_ray(rx, oracle_mod, gpu_ctx, n):
+    """Batch sizes around the work distribution's boundaries: queues short
+    enough for static chunks alone (no counter), one chunk per wave exactly,
+    and long ones whose counter part is cut into eight segments with ragged
+    ends.  Every ray must get its own result: the exact tid and distance of
+    the oracle recursion.  The same rays are first queried reversed, so a
+    ray the second query skipped would keep the reversed ray's result."""
+    print(f"[work distribution] n={n}", flush=True)
+    sc = scenes.cornell(8, 8)
+    tree = rx.KDTree(sc.verts, sc.tris)
+    ig = rx.IntersectionGPU(ctx=gpu_ctx)
+    ig.loadKDTreeToShader(tree)
+    rng = np.random.default_rng(0x52454E44 + n)  # origins inside the box: forward and reversed results differ
+    o = rng.uniform(0.05, 0.95, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    ig.query(o, -d)
+    g = ig.query(o, d)
+    ot = oracle_mod.Tree(sc.verts, sc.tris, 0)
+    e_dist, e_tid, _, _ = ot.intersect(o, d, 0)
+    assert np.array_equal(g["tid"], e_tid) and _same_bits(g["dist"], e_dist)
+
+
 def test_full_size_icosphere_2p22_rays(rx, oracle_mod, gpu_ctx):
     """The 1,003,526-triangle scene and 2^22 rays: ordered == exhaustive on the
     whole batch, and a 65,536-ray subset against the oracle recursion."""
