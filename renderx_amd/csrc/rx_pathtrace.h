@@ -1294,27 +1294,68 @@ static __global__ void k_guide_gather(TrainOut S, const uint32_t* total, uint32_
 // Adds the pass's samples of each pixel in sample order (slot pass_slot(q, j)
 // holds sample `sample + j` of pixel q), so the FP64 film is independent of
 // how samples are grouped into passes.
-static __global__ void k_accumulate(PathBufs B, PassArgs P) {
+constexpr uint32_t kAccPix = kRBlock;  // pixels per block of k_accumulate, one per thread
+constexpr uint32_t kAccChunk = 8;      // samples per pixel staged in LDS at a time
+static __global__ void __launch_bounds__(kAccPix) k_accumulate(PathBufs B, PassArgs P) {
   const uint32_t npix = P.npix, S = P.spass;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < npix; q += stride) {
-    double s[3], s2[3];
-    for (int k = 0; k < 3; k++) {
-      s[k] = B.sum[3 * (size_t)q + k];
-      s2[k] = B.sumsq[3 * (size_t)q + k];
-    }
-    for (uint32_t j = 0; j < S; j++) {
-      const float4 L = B.L[pass_slot(P, q, j)];
-      const double l[3] = {L.x, L.y, L.z};
+  if (!RX_PIXEL_MAJOR) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < npix; q += gridDim.x * blockDim.x) {
+      double s[3], s2[3];
       for (int k = 0; k < 3; k++) {
-        s[k] += l[k];
-        s2[k] += l[k] * l[k];
+        s[k] = B.sum[3 * (size_t)q + k];
+        s2[k] = B.sumsq[3 * (size_t)q + k];
+      }
+      for (uint32_t j = 0; j < S; j++) {
+        const float4 L = B.L[pass_slot(P, q, j)];
+        const double l[3] = {L.x, L.y, L.z};
+        for (int k = 0; k < 3; k++) {
+          s[k] += l[k];
+          s2[k] += l[k] * l[k];
+        }
+      }
+      for (int k = 0; k < 3; k++) {
+        B.sum[3 * (size_t)q + k] = s[k];
+        B.sumsq[3 * (size_t)q + k] = s2[k];
       }
     }
-    for (int k = 0; k < 3; k++) {
-      B.sum[3 * (size_t)q + k] = s[k];
-      B.sumsq[3 * (size_t)q + k] = s2[k];
+    return;
+  }
+  // Pixel-major slots: a block's pixels hold one contiguous run of slots.
+  // It is read kAccChunk samples per pixel at a time into LDS, eight
+  // neighbouring lanes on one pixel's neighbouring samples (coalesced), and
+  // each thread then adds its own pixel's samples in order.
+  __shared__ float4 tile[kAccPix * (kAccChunk + 1)];  // one float4 of padding per pixel row
+  for (uint32_t q0 = blockIdx.x * kAccPix; q0 < npix; q0 += gridDim.x * kAccPix) {  // block-uniform
+    const uint32_t q = q0 + threadIdx.x;
+    double s[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+    if (q < npix)
+      for (int k = 0; k < 3; k++) {
+        s[k] = B.sum[3 * (size_t)q + k];
+        s2[k] = B.sumsq[3 * (size_t)q + k];
+      }
+    for (uint32_t j0 = 0; j0 < S; j0 += kAccChunk) {
+      const uint32_t nj = min(kAccChunk, S - j0);
+      for (uint32_t e = threadIdx.x; e < kAccPix * kAccChunk; e += blockDim.x) {
+        const uint32_t r = e / kAccChunk, t = e - r * kAccChunk;
+        if (q0 + r < npix && t < nj) tile[r * (kAccChunk + 1) + t] = B.L[(size_t)(q0 + r) * S + j0 + t];
+      }
+      __syncthreads();
+      if (q < npix)
+        for (uint32_t t = 0; t < nj; t++) {
+          const float4 L = tile[threadIdx.x * (kAccChunk + 1) + t];
+          const double l[3] = {L.x, L.y, L.z};
+          for (int k = 0; k < 3; k++) {
+            s[k] += l[k];
+            s2[k] += l[k] * l[k];
+          }
+        }
+      __syncthreads();
     }
+    if (q < npix)
+      for (int k = 0; k < 3; k++) {
+        B.sum[3 * (size_t)q + k] = s[k];
+        B.sumsq[3 * (size_t)q + k] = s2[k];
+      }
   }
 }
 

@@ -331,7 +331,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
         if (mode) sdw.push_back({e2, e3});
       }
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(kRBlock), 0, s, B, P);
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(kAccPix), 0, s, B, P);
     RXCK(hipGetLastError(), "launch pass");
     if (GA.tag > 0.0f) {  // online training on this pass's radiance samples
       GuideState& G = *c->guide;
