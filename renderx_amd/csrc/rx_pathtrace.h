@@ -754,8 +754,15 @@ __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3
   }
 }
 
+// minimum resident waves per SIMD for the plain medium-free shade kernel
+// (97 VGPRs unconstrained; 6 waves spill 15 and beat 4 and 5 by about 0.5 %
+// per bench step).  The other variants need far more registers and keep
+// their own allocation.
+#ifndef RX_SHADE_MIN_WAVES
+#define RX_SHADE_MIN_WAVES 6
+#endif
 template <int MODE, bool BATCH, bool VOL>
-__global__ void __launch_bounds__(kRBlock) k_shade(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
+__global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WAVES : 1) k_shade(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
                                                    BatchArgs BA) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
   __shared__ uint32_t cont_list[kStageCap], nee_list[MODE >= 1 ? kStageCap : 1], counts[4];
