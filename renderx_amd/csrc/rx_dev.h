@@ -293,9 +293,7 @@ __device__ __forceinline__ bool trace_begin(const TreeRoot& R, TraceState& s, Ve
 //
 // lds_stack: per-lane LDS column of LDS_DEPTH entries (stride = blockDim.x),
 // spill: per-thread global overflow of spill_depth entries.
-// FAST_ONLY: the caller only passes rays the slab filter serves (s.fast),
-// and the exact-predicate-only path is left out of the loop.
-template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, bool ANYHIT = true, bool FAST_ONLY = false>
+template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, bool ANYHIT = true>
 __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint2* lds_stack, uint2* spill,
                                            int spill_depth, TraceStats* st, bool* overflow) {
   const float INF = __builtin_inff();
@@ -310,7 +308,7 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
     if (STATS) st->boxes += 2;
     bool hl, hr;
     float tl = 0.0f, tr = 0.0f;
-    if (FAST_ONLY || s.fast) {  // the exact predicate only where the slab filter cannot decide
+    if (s.fast) {  // the exact predicate only where the slab filter cannot decide
       const int cl = slab_classify(llo, lhi, s.o, s.inv, s.beta, tl);
       const int cr = slab_classify(rlo, rhi, s.o, s.inv, s.beta, tr);
       // a child the ordered traversal culls (entry beyond the cull distance)
@@ -525,24 +523,12 @@ __device__ __forceinline__ void trace_range(const DevTree& T, uint32_t n, uint32
         TreeRoot root = tree_root(T);
         fetch(i, o, d, ins, con, root);
         active = trace_begin<STATS>(root, ts, o, d, ins, con, &st);
-        if (active && !ts.fast) {
-          // a ray the slab filter cannot serve (a direction component below
-          // 2^-30, e.g. exactly axis-parallel): its whole traversal here, on
-          // the exact predicate alone, so that the shared loop below needs
-          // no code for it
-          while (!trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH, false>(T, ts, lds_stack, spill, spill_depth, &st,
-                                                                           overflow)) {
-          }
-          emit(i, ts.has ? Hit{ts.best, ts.bestPos} : Hit{-1.0f, -1}, st);
-          active = false;
-        } else if (!active) {
-          emit(i, Hit{-1.0f, -1}, st);  // missed the scene box
-        }
+        if (!active) emit(i, Hit{-1.0f, -1}, st);  // missed the scene box
       }
     }
     if (w.drained && !__any(active)) break;
-    if (active && trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH, false, true>(T, ts, lds_stack, spill, spill_depth,
-                                                                                 &st, overflow)) {
+    if (active &&
+        trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH, false>(T, ts, lds_stack, spill, spill_depth, &st, overflow)) {
       emit(cur, ts.has ? Hit{ts.best, ts.bestPos} : Hit{-1.0f, -1}, st);
       active = false;
     }
