@@ -154,6 +154,12 @@ int rx_create(int device, rx_ctx** out);
 void rx_destroy(rx_ctx* ctx);
 const char* rx_last_error(const rx_ctx* ctx);
 int rx_synchronize(rx_ctx* ctx);
+/* Test hook, no counterpart in the reference: the nth device allocation the
+ * library makes from now on (0 = the next one) fails as out of memory, once;
+ * negative switches it off.  Process-wide.  It shows that a load interrupted
+ * by a failed allocation leaves the context without a scene or tree
+ * (RX_ERR_STATE from rx_render / rx_intersect) instead of half-built. */
+int rx_debug_fail_device_alloc(int32_t nth);
 
 /* IntersectionGPU::loadKDTreeToShader (src/IntersectionGPU.cpp:243-255):
  * takes the arrays KDTree::serializeForGPU produces, unchanged.  Replaces
@@ -175,16 +181,22 @@ enum rx_intersect_flags {
 /* IntersectionGPU::query (src/IntersectionGPU.cpp:216-241) with the result of
  * KDTree::intersect(ray, triangleID, condition) (src/KDTree.cpp:335-339):
  * org/dir 3n floats; inside/contact n object ids (-1 = none), used with
- * RX_ISECT_LEGALITY, else may be NULL.  Outputs: dist (-1 = miss), tid (-1 =
- * miss), optional bary (2n, FP32 of an FP64 solve: p = (1-u-v)v0+u v1+v v2),
+ * RX_ISECT_LEGALITY, else may be NULL; skip_tid (n triangle ids, -1 = none,
+ * may be NULL) is IntersectionGPU::Ray::last_tid (include/IntersectionGPU.h:14-25):
+ * the named triangle is passed over, as the shader does (Shader/Intersection.frag:63-64)
+ * for the re-test rays of Scene::fillIntersectObject / testVisibility
+ * (src/Scene.cpp:438,463,487).  Outputs: dist (-1 = miss), tid (-1 = miss),
+ * optional bary (2n, FP32 of an FP64 solve: p = (1-u-v)v0+u v1+v v2),
  * optional counts (2n: box tests, triangle tests). */
 int rx_intersect(rx_ctx* ctx, uint32_t n, const float* org, const float* dir, const int32_t* inside,
-                 const int32_t* contact, uint32_t flags, float* dist, int32_t* tid, float* bary,
-                 uint32_t* counts);
-/* Same with device pointers, queued on the context stream. */
+                 const int32_t* contact, const int32_t* skip_tid, uint32_t flags, float* dist, int32_t* tid,
+                 float* bary, uint32_t* counts);
+/* Same with device pointers, queued on the context stream.  A traversal
+ * stack overflow of a queued call is reported by the next rx_synchronize
+ * (RX_ERR_OVERFLOW). */
 int rx_intersect_dev(rx_ctx* ctx, uint32_t n, const float* org, const float* dir, const int32_t* inside,
-                     const int32_t* contact, uint32_t flags, float* dist, int32_t* tid, float* bary,
-                     uint32_t* counts);
+                     const int32_t* contact, const int32_t* skip_tid, uint32_t flags, float* dist, int32_t* tid,
+                     float* bary, uint32_t* counts);
 /* Duration in ms of the last rx_intersect(_dev) traversal kernel (HIP events
  * on the context stream; valid after rx_synchronize). */
 int rx_last_kernel_ms(rx_ctx* ctx, float* ms);

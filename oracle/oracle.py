@@ -41,7 +41,7 @@ def lib():
         L.orc_tree_sizes.argtypes = [P, P, P]
         L.orc_tree_serialize.argtypes = [P] * 7
         L.orc_face_normals.argtypes = [C.c_int, P, C.c_int, P, P]
-        L.orc_intersect.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, C.c_int]
+        L.orc_intersect.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P, C.c_int]
         L.orc_barycentrics.argtypes = [C.c_int, P, P, P, P, P, P]
         L.orc_scene_create.restype = P
         L.orc_scene_create.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P, P, C.c_int]
@@ -54,6 +54,11 @@ def lib():
         L.orc_mt_nth.argtypes = [C.c_uint32]
         L.orc_max_threads.restype = C.c_int
         L.orc_sampler_draws.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, P]
+        L.orc_set_mutation.argtypes = [C.c_int]
+        L.orc_refract_dir.restype = C.c_int
+        L.orc_refract_dir.argtypes = [C.c_int, P, P, C.c_float, C.c_float, P, P]
+        L.orc_medium_densities.argtypes = [P, C.c_float, P, P]
+        L.orc_glossy_eval.argtypes = [P, P, P, P, C.c_float, P]
         _lib = L
     return _lib
 
@@ -90,8 +95,9 @@ class Tree:
         return out
 
     def intersect(self, org, dirs, mode=0, tri_obj=None, tri_normal=None, inside=None, contact=None,
-                  threads=0):
-        """mode 0 exhaustive recursion, 1 ordered (canonical counts), 2 brute force."""
+                  threads=0, skip=None):
+        """mode 0 exhaustive recursion, 1 ordered (canonical counts), 2 brute force.
+        skip: per-ray triangle id never hit (IntersectionGPU::Ray::last_tid), -1 = none."""
         org = np.ascontiguousarray(org, np.float32)
         dirs = np.ascontiguousarray(dirs, np.float32)
         n = len(org)
@@ -101,9 +107,9 @@ class Tree:
         tris = np.zeros(n, np.uint32)
         cv = lambda a, t: None if a is None else np.ascontiguousarray(a, t)
         tri_obj, tri_normal = cv(tri_obj, np.int32), cv(tri_normal, np.float32)
-        inside, contact = cv(inside, np.int32), cv(contact, np.int32)
+        inside, contact, skip = cv(inside, np.int32), cv(contact, np.int32), cv(skip, np.int32)
         lib().orc_intersect(self.h, mode, n, _p(org), _p(dirs), _p(tri_obj), _p(tri_normal), _p(inside),
-                            _p(contact), _p(dist), _p(tid), _p(boxes), _p(tris), threads)
+                            _p(contact), _p(skip), _p(dist), _p(tid), _p(boxes), _p(tris), threads)
         return dist, tid, boxes, tris
 
 
@@ -193,6 +199,52 @@ def sampler_draws(kind: int, n: int, param: float, param2: float = 0.0, seed: in
     """kind 0 cosine lobe, 1 HG phase, 2 medium distance, 3 Fresnel er; (n,3) float32."""
     out = np.empty((n, 3), np.float32)
     lib().orc_sampler_draws(kind, n, param, param2, seed, _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- single functions, for their pins
+# Planted mistakes (see rx_oracle_scene.h): each must make one pin fail.
+MUT_NONE, MUT_ROTATION_SIGN, MUT_INVERTED_RATIO, MUT_P_SURFACE_COMPLEMENT, MUT_P_MEDIUM_NO_SIGMA, MUT_GLOSSY_AXIS = range(6)
+
+
+class mutation:
+    """Context manager: the oracle computes with a planted mistake inside the block."""
+
+    def __init__(self, m: int):
+        self.m = m
+
+    def __enter__(self):
+        lib().orc_set_mutation(self.m)
+
+    def __exit__(self, *exc):
+        lib().orc_set_mutation(0)
+
+
+def refract_dir(d, n, n_i, n_t, kind=0):
+    """The oracle's Snell/Fresnel step: kind 0 RefractiveMaterial, 1 SceneVPMObject.
+    Returns (ok, direction (3,) float32, er); ok False = total internal reflection."""
+    d = np.ascontiguousarray(d, np.float32)
+    n = np.ascontiguousarray(n, np.float32)
+    out = np.zeros(3, np.float32)
+    er = np.zeros(1, np.float32)
+    ok = lib().orc_refract_dir(kind, _p(d), _p(n), n_i, n_t, _p(out), _p(er))
+    return bool(ok), out, float(er[0])
+
+
+def medium_densities(sigma_t, x):
+    """(p_medium(x), P_surface(x)) of a homogeneous medium with extinction sigma_t (3,)."""
+    st = np.ascontiguousarray(sigma_t, np.float32)
+    a = np.zeros(1, np.float32)
+    b = np.zeros(1, np.float32)
+    lib().orc_medium_densities(_p(st), x, _p(a), _p(b))
+    return float(a[0]), float(b[0])
+
+
+def glossy_eval(n, d_in, d_out, ks, exponent):
+    """GlossyBSDF::evaluate restated: colour (3,) of the lobe for incoming d_in, outgoing d_out."""
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (n, d_in, d_out, ks)]
+    out = np.zeros(3, np.float32)
+    lib().orc_glossy_eval(*[_p(a) for a in arrs], exponent, _p(out))
     return out
 
 

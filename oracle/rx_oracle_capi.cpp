@@ -142,15 +142,17 @@ void orc_face_normals(int nv, const float* v, int nt, const uint32_t* tri, float
 
 // Nearest hit for a batch.  mode 0 = exhaustive recursion (reference),
 // 1 = ordered traversal (canonical counts), 2 = brute force.
-// Legality arrays may be NULL (no condition).
+// Legality arrays may be NULL (no condition); skip (may be NULL) names a
+// triangle id per ray that is never hit (IntersectionGPU::Ray::last_tid).
 void orc_intersect(void* h, int mode, int n, const float* org, const float* dir, const int* tri_obj,
-                   const float* tri_normal, const int* inside, const int* contact, float* dist, int* tid,
-                   uint32_t* boxes, uint32_t* tris, int threads) {
+                   const float* tri_normal, const int* inside, const int* contact, const int* skip, float* dist,
+                   int* tid, uint32_t* boxes, uint32_t* tris, int threads) {
   const Tree& t = *(Tree*)h;
   if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 256)
   for (int i = 0; i < n; i++) {
-    Ray r{V3(org[3 * i], org[3 * i + 1], org[3 * i + 2]), V3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2])};
+    Ray r{V3(org[3 * i], org[3 * i + 1], org[3 * i + 2]), V3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
+          skip ? skip[i] : -1};
     Legality L{tri_obj, (const V3*)tri_normal, inside ? inside[i] : -1, contact ? contact[i] : -1};
     const Legality* cond = (tri_obj && tri_normal && inside && contact) ? &L : nullptr;
     HitStats st;
@@ -246,5 +248,16 @@ void orc_sampler_draws(int kind, int n, float param, float param2, uint64_t seed
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
+
+void orc_set_mutation(int m) { g_mutation = m; }
+int orc_refract_dir(int kind, const float* d, const float* n, float n_i, float n_t, float* dir, float* er) {
+  return pin_refract(kind, d, n, n_i, n_t, dir, er) ? 1 : 0;
+}
+void orc_medium_densities(const float* sigma_t, float x, float* p_med, float* P_surf) {
+  pin_medium(sigma_t, x, p_med, P_surf);
+}
+void orc_glossy_eval(const float* n, const float* in, const float* out, const float* ks, float coeff, float* res) {
+  pin_glossy(n, in, out, ks, coeff, res);
+}
 
 }  // extern "C"

@@ -201,6 +201,7 @@ bool legal(const Legality& c, const Ray& r, unsigned tid) {
 }
 
 static inline float tri_test(const Tree& t, unsigned tr, const Ray& r, const Legality* cond) {
+  if ((int)tr == r.skip) return -1;
   float dist = tri_intersect(t.verts[t.tri_v[3 * tr]], t.verts[t.tri_v[3 * tr + 1]],
                              t.verts[t.tri_v[3 * tr + 2]], r);
   if (dist < 0) return -1;  // geometric miss returns -1 exactly

@@ -92,6 +92,7 @@ void serialize(const Tree& t, Serialized& s);
 
 struct Ray {
   V3 o, d;
+  int skip = -1;  // IntersectionGPU::Ray::last_tid: this triangle id is never hit (Shader/Intersection.frag:63-64)
 };
 
 // KDTree::Node::BoundingBox::intersect (src/KDTree.cpp:34-52)

@@ -26,6 +26,9 @@
 namespace orc {
 
 static const float PI_F = (float)M_PI;
+// Planted mistakes for the tests that show the oracle's pins catch them
+// (orc_set_mutation); 0 in every other use.
+int g_mutation = 0;
 // traversal queries issued by this thread (camera, bounce, shadow, containment)
 static thread_local uint64_t g_rays = 0;
 static const float EPS = 1e-5f;          // EPSILON (include/macros.h:7)
@@ -122,6 +125,7 @@ static inline float cos_pdf(const Frame& f, V3 dir, float coeff) {
 // rotMat(axis, angle) * vec4(v, 0) (include/smallFuncs.h:11-24; the transpose
 // of the column-major constructor is the right-handed rotation by +angle)
 static V3 rotate(V3 axis, float angle, V3 v) {
+  if (g_mutation == MUT_ROTATION_SIGN) angle = -angle;
   float c = std::cos(angle), s = std::sin(angle), _c = 1 - c;
   float x = axis.x, y = axis.y, z = axis.z;
   float R[3][3] = {{c + _c * x * x, _c * x * y - s * z, _c * x * z + s * y},
@@ -130,6 +134,31 @@ static V3 rotate(V3 axis, float angle, V3 v) {
   V3 r;
   for (int i = 0; i < 3; i++) r[i] = v.x * R[i][0] + v.y * R[i][1] + v.z * R[i][2] + 0.0f * 0.0f;
   return r;
+}
+
+// The Snell / Fresnel step of RefractiveMaterial::scatter
+// (src/RefractiveMaterial.cpp:189-232) and SceneVPMObject's boundary crossings
+// (src/SceneVPMObject.cpp:44-75,176-209): theta = acos(d.n) (clamped to
+// [-1, 1] in RefractiveMaterial only), sin(phi) = n_i / n_t sin(theta); false
+// on total internal reflection; otherwise the normal rotated by phi about
+// normalize(n x d) (phi -> pi - phi when the ray runs against the normal) and
+// the unpolarised Fresnel reflectance er = (r_s^2 + r_p^2) / 2.
+static bool refr_dir(V3 d, V3 normal, float current_n, float next_n, bool clamp_cos, V3& dir, float& er) {
+  if (g_mutation == MUT_INVERTED_RATIO) std::swap(current_n, next_n);
+  float cd = dot(d, normal);
+  if (clamp_cos) cd = cd < -1 ? -1.f : cd > 1 ? 1.f : cd;
+  float theta = std::acos(cd);
+  float sin_phi = current_n / next_n * std::sin(theta);
+  if (sin_phi >= 1) return false;
+  float phi = std::asin(sin_phi);
+  if (theta > M_PI / 2) phi = M_PI - phi;
+  V3 axis = normalize(cross(normal, d));
+  dir = normalize(rotate(axis, phi, normal));
+  float cos_theta = std::fabs(std::cos(theta)), cos_phi = std::fabs(std::cos(phi));
+  float esr = std::pow(std::fabs(current_n * cos_theta - next_n * cos_phi) / (current_n * cos_theta + next_n * cos_phi), 2.f);
+  float epr = std::pow(std::fabs(next_n * cos_theta - current_n * cos_phi) / (next_n * cos_theta + current_n * cos_phi), 2.f);
+  er = (esr + epr) / 2;
+  return true;
 }
 
 // ---------------------------------------------------------------- Ray record
@@ -262,7 +291,7 @@ static inline V3 mirror(V3 n, V3 d) { return -dot(n, d) * n * 2.0f + d; }
 // GlossyBSDF::evaluate (include/GlossyBSDF.h:24-33)
 static V3 glossy_eval(const Frame& lf, V3 in, V3 out, V3 color, float coeff) {
   if (dot(lf.n, in) <= 0 && dot(lf.n, out) >= 0) {
-    V3 r = mirror(lf.n, in);
+    V3 r = g_mutation == MUT_GLOSSY_AXIS ? -in : mirror(lf.n, in);
     if (dot(r, out) > 0) return color * std::pow(dot(r, out), coeff);
   }
   return V3(0, 0, 0);
@@ -325,8 +354,6 @@ static PRay refractive_scatter(Ctx& c, int object, const PRay& in, bool russian)
   out.directionSampleType = DEFINITE;
   if (in.intersectObject != object && in.insideObject == object)
     return scatter(c, in.intersectObject, in, russian);
-  float cd = dot(in.direction, normal);
-  float theta = std::acos(cd < -1 ? -1.f : cd > 1 ? 1.f : cd);
   int currentInside = in.insideObject;
   int outSide = object;
   if (in.insideObject == object) outSide = find_inside(c.sc, out.origin, out.direction, object);
@@ -334,21 +361,14 @@ static PRay refractive_scatter(Ctx& c, int object, const PRay& in, bool russian)
   float next_n = refr_coeff(c.sc, outSide);
   out.intersectObject = NONE;
   out.directionProb = 1;
-  float sin_phi = current_n / next_n * std::sin(theta);
-  if (sin_phi >= 1) {  // total internal reflection terminates the path
+  float er;
+  if (!refr_dir(in.direction, normal, current_n, next_n, true, out.direction, er)) {
+    // total internal reflection terminates the path
     out.direction = V3(0, 0, 0);
     out.contactObject = NONE;
     out.directionSampleType = DEFINITE;
     return out;
   }
-  float phi = std::asin(sin_phi);
-  if (theta > M_PI / 2) phi = M_PI - phi;
-  V3 axis = normalize(cross(normal, in.direction));
-  out.direction = normalize(rotate(axis, phi, normal));
-  float cos_theta = std::fabs(std::cos(theta)), cos_phi = std::fabs(std::cos(phi));
-  float esr = std::pow(std::fabs(current_n * cos_theta - next_n * cos_phi) / (current_n * cos_theta + next_n * cos_phi), 2.f);
-  float epr = std::pow(std::fabs(next_n * cos_theta - current_n * cos_phi) / (next_n * cos_theta + current_n * cos_phi), 2.f);
-  float er = (esr + epr) / 2;
   float p = er < 0.f ? 0.f : er > 1.f ? 1.f : er;
   if (c.g.f() < p) {
     out.direction = reflDir;
@@ -392,28 +412,26 @@ static inline float hg_value(float g, float cosTheta) {
 static bool vpm_refract(Ctx& c, const PRay& in, V3 normal, float current_n, float next_n, PRay& out,
                         float& er, V3& reflDir) {
   reflDir = normalize(mirror(normal, in.direction));
-  float theta = std::acos(dot(in.direction, normal));
-  float sin_phi = current_n / next_n * std::sin(theta);
-  if (sin_phi >= 1) return false;
-  float phi = std::asin(sin_phi);
-  if (theta > M_PI / 2) phi = M_PI - phi;
-  V3 axis = normalize(cross(normal, in.direction));
-  out.direction = normalize(rotate(axis, phi, normal));
-  float cos_theta = std::fabs(std::cos(theta)), cos_phi = std::fabs(std::cos(phi));
-  float esr = std::pow(std::fabs(current_n * cos_theta - next_n * cos_phi) / (current_n * cos_theta + next_n * cos_phi), 2.f);
-  float epr = std::pow(std::fabs(next_n * cos_theta - current_n * cos_phi) / (next_n * cos_theta + current_n * cos_phi), 2.f);
-  er = (esr + epr) * 0.5f;
-  return true;
+  return refr_dir(in.direction, normal, current_n, next_n, false, out.direction, er);
 }
 
-static float p_medium(const ObjO& o, float s) {  // SceneVPMObject::p_medium
-  V3 pm(o.sigma_t.x * std::exp(-o.sigma_t.x * s), o.sigma_t.y * std::exp(-o.sigma_t.y * s),
-        o.sigma_t.z * std::exp(-o.sigma_t.z * s));
+// SceneVPMObject::p_medium (include/SceneVPMObject.h:39-47): density of
+// scattering at distance s, y(sigma_t e^{-sigma_t s})
+static float p_medium(V3 sigma_t, float s) {
+  if (g_mutation == MUT_P_MEDIUM_NO_SIGMA)
+    return lum(V3(std::exp(-sigma_t.x * s), std::exp(-sigma_t.y * s), std::exp(-sigma_t.z * s)));
+  V3 pm(sigma_t.x * std::exp(-sigma_t.x * s), sigma_t.y * std::exp(-sigma_t.y * s), sigma_t.z * std::exp(-sigma_t.z * s));
   return lum(pm);
 }
-static float P_surface(const ObjO& o, float d) {  // SceneVPMObject::P_surface
-  return lum(V3(std::exp(-o.sigma_t.x * d), std::exp(-o.sigma_t.y * d), std::exp(-o.sigma_t.z * d)));
+// SceneVPMObject::P_surface (include/SceneVPMObject.h:27-37): probability of
+// reaching the surface at distance d, y(e^{-sigma_t d})
+static float P_surface(V3 sigma_t, float d) {
+  if (g_mutation == MUT_P_SURFACE_COMPLEMENT)
+    return lum(V3(1 - std::exp(-sigma_t.x * d), 1 - std::exp(-sigma_t.y * d), 1 - std::exp(-sigma_t.z * d)));
+  return lum(V3(std::exp(-sigma_t.x * d), std::exp(-sigma_t.y * d), std::exp(-sigma_t.z * d)));
 }
+static float p_medium(const ObjO& o, float s) { return p_medium(o.sigma_t, s); }
+static float P_surface(const ObjO& o, float d) { return P_surface(o.sigma_t, d); }
 
 // SceneVPMObject::scatter (src/SceneVPMObject.cpp:4-251)
 static PRay vpm_scatter(Ctx& c, int self, const PRay& in, bool russian) {
@@ -1160,6 +1178,31 @@ void sampler_draws(int kind, int n, float param, float param2, uint64_t seed, fl
     out[3 * i + 1] = r.y;
     out[3 * i + 2] = r.z;
   }
+}
+
+bool pin_refract(int kind, const float* d, const float* n, float n_i, float n_t, float* dir, float* er) {
+  V3 out(0, 0, 0);
+  float e = 0;
+  const bool ok = refr_dir(V3(d[0], d[1], d[2]), V3(n[0], n[1], n[2]), n_i, n_t, kind == 0, out, e);
+  dir[0] = out.x;
+  dir[1] = out.y;
+  dir[2] = out.z;
+  *er = e;
+  return ok;
+}
+
+void pin_medium(const float* sigma_t, float x, float* p_med, float* P_surf) {
+  const V3 st(sigma_t[0], sigma_t[1], sigma_t[2]);
+  *p_med = p_medium(st, x);
+  *P_surf = P_surface(st, x);
+}
+
+void pin_glossy(const float* n, const float* in, const float* out, const float* ks, float coeff, float* res) {
+  const Frame lf = frame_from(V3(n[0], n[1], n[2]));
+  const V3 r = glossy_eval(lf, V3(in[0], in[1], in[2]), V3(out[0], out[1], out[2]), V3(ks[0], ks[1], ks[2]), coeff);
+  res[0] = r.x;
+  res[1] = r.y;
+  res[2] = r.z;
 }
 
 uint32_t mt_nth(uint32_t n) {

@@ -63,6 +63,22 @@ struct SceneO {
 };
 
 void render(const SceneO& sc, const RenderParamsO& p, double* sum, double* sumsq, uint64_t* rays);
+
+// Planted mistakes (test hooks only): each must make one of the oracle's pins fail.
+enum Mutation {
+  MUT_NONE = 0,
+  MUT_ROTATION_SIGN = 1,         // rotMat by -phi: the refracted ray bends the wrong way
+  MUT_INVERTED_RATIO = 2,        // n_t / n_i instead of n_i / n_t
+  MUT_P_SURFACE_COMPLEMENT = 3,  // P_surface = y(1 - e^{-sigma_t d})
+  MUT_P_MEDIUM_NO_SIGMA = 4,     // p_medium without its sigma_t factor
+  MUT_GLOSSY_AXIS = 5            // glossy lobe about -d instead of the mirror direction
+};
+extern int g_mutation;
+// The oracle's own functions, for the tests that pin them:
+// kind 0 = RefractiveMaterial's step, 1 = SceneVPMObject's; false on TIR.
+bool pin_refract(int kind, const float* d, const float* n, float n_i, float n_t, float* dir, float* er);
+void pin_medium(const float* sigma_t, float x, float* p_med, float* P_surf);
+void pin_glossy(const float* n, const float* in, const float* out, const float* ks, float coeff, float* res);
 uint32_t mt_nth(uint32_t n);
 void sampler_draws(int kind, int n, float param, float param2, uint64_t seed, float* out);
 

@@ -30,10 +30,11 @@ SIGNATURES = {
     "rx_destroy": (None, [P]),
     "rx_last_error": (C.c_char_p, [P]),
     "rx_synchronize": (C.c_int, [P]),
+    "rx_debug_fail_device_alloc": (C.c_int, [I32]),
     "rx_load_tree": (C.c_int, [P, U32, P, P, P, U32, P, P, P]),
     "rx_load_legality": (C.c_int, [P, U32, P, P]),
-    "rx_intersect": (C.c_int, [P, U32, P, P, P, P, U32, P, P, P, P]),
-    "rx_intersect_dev": (C.c_int, [P, U32, P, P, P, P, U32, P, P, P, P]),
+    "rx_intersect": (C.c_int, [P, U32, P, P, P, P, P, U32, P, P, P, P]),
+    "rx_intersect_dev": (C.c_int, [P, U32, P, P, P, P, P, U32, P, P, P, P]),
     "rx_last_kernel_ms": (C.c_int, [P, C.POINTER(F32)]),
     "rx_copy_kernel": (C.c_int, [P, P, P, U64]),
     "rx_load_scene": (C.c_int, [P, U32, P, U32, P, U32, P, U32]),

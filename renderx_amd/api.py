@@ -94,7 +94,10 @@ class IntersectionGPU:
         tn = None if tri_normal is None else np.ascontiguousarray(tri_normal, np.float32)
         self.ctx.check(lib().rx_load_legality(self.ctx.h, len(tri_obj), ptr(tri_obj), ptr(tn)), "rx_load_legality")
 
-    def query(self, org, dirs, inside=None, contact=None, exhaustive=False, bary=False, counts=False):
+    def query(self, org, dirs, inside=None, contact=None, exhaustive=False, bary=False, counts=False,
+              last_tid=None):
+        """IntersectionGPU::query: nearest hit per ray.  last_tid (per ray, -1 =
+        none) is the triangle the ray never hits, IntersectionGPU::Ray::last_tid."""
         org = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
         dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
         n = len(org)
@@ -108,8 +111,9 @@ class IntersectionGPU:
         tid = np.empty(n, np.int32)
         b = np.empty((n, 2), np.float32) if bary else None
         c = np.empty((n, 2), np.uint32) if counts else None
-        self.ctx.check(lib().rx_intersect(self.ctx.h, n, ptr(org), ptr(dirs), ptr(ins), ptr(con), flags, ptr(dist),
-                                          ptr(tid), ptr(b), ptr(c)), "rx_intersect")
+        skip = None if last_tid is None else np.ascontiguousarray(last_tid, np.int32)
+        self.ctx.check(lib().rx_intersect(self.ctx.h, n, ptr(org), ptr(dirs), ptr(ins), ptr(con), ptr(skip), flags,
+                                          ptr(dist), ptr(tid), ptr(b), ptr(c)), "rx_intersect")
         out = {"dist": dist, "tid": tid}
         if bary:
             out["bary"] = b

@@ -15,6 +15,8 @@ struct rx_tree {
 
 namespace rx {
 
+int g_fail_alloc_countdown = -1;
+
 int set_err(rx_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
@@ -86,6 +88,7 @@ int upload_tree(rx_ctx* c) {
       memcpy(&dh.tn[4 * (size_t)k], &c->tri_normal[3 * (size_t)tid], 12);
     }
   }
+  c->tree_loaded = false;  // the device tree is incomplete until every upload has succeeded
   if (int r = hip_err(c, c->tree.upload(dh, c->host_tree.max_depth, c->stream), "tree upload")) return r;
   c->tree_loaded = true;
   return RX_OK;
@@ -212,9 +215,21 @@ void rx_destroy(rx_ctx* c) {
 
 const char* rx_last_error(const rx_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+int rx_debug_fail_device_alloc(int32_t nth) {
+  g_fail_alloc_countdown = nth < 0 ? -1 : nth;
+  return RX_OK;
+}
+
 int rx_synchronize(rx_ctx* c) {
   if (!c) return RX_ERR_INVALID;
-  return hip_err(c, hipStreamSynchronize(c->stream), "synchronize");
+  if (int r = hip_err(c, hipStreamSynchronize(c->stream), "synchronize")) return r;
+  if (c->isect_queued && c->overflow.p) {  // the traversal stack flag of the queued rx_intersect_dev calls
+    c->isect_queued = false;
+    int32_t ovf = 0;
+    if (int r = hip_err(c, hipMemcpy(&ovf, c->overflow.p, 4, hipMemcpyDeviceToHost), "overflow flag")) return r;
+    if (ovf) return set_err(c, RX_ERR_OVERFLOW, "rx_intersect_dev: traversal stack overflow");
+  }
+  return RX_OK;
 }
 
 int rx_load_tree(rx_ctx* c, uint32_t n_nodes, const float* nodes4, const float* min4, const float* max4,
@@ -222,28 +237,41 @@ int rx_load_tree(rx_ctx* c, uint32_t n_nodes, const float* nodes4, const float* 
   if (!c || (n_nodes && (!nodes4 || !min4 || !max4)) || (n_leaf_tris && (!v1 || !v2 || !v3)))
     return set_err(c, RX_ERR_INVALID, "rx_load_tree: null array");
   (void)hipSetDevice(c->device);
-  if (!parse_serialized(n_nodes, nodes4, min4, max4, n_leaf_tris, v1, v2, v3, c->host_tree, c->leaf_verts))
-    return set_err(c, RX_ERR_INVALID, "rx_load_tree: malformed serialized tree");
-  c->scene_loaded = false;
-  c->tri_obj.clear();  // legality data belongs to the previous tree
-  c->tri_normal.clear();
-  return upload_tree(c);
+  c->scene_loaded = false;  // the previous scene and tree are gone from here on
+  c->tree_loaded = false;
+  try {
+    if (!parse_serialized(n_nodes, nodes4, min4, max4, n_leaf_tris, v1, v2, v3, c->host_tree, c->leaf_verts))
+      return set_err(c, RX_ERR_INVALID, "rx_load_tree: malformed serialized tree");
+    c->tri_obj.clear();  // legality data belongs to the previous tree
+    c->tri_normal.clear();
+    return upload_tree(c);
+  } catch (const std::bad_alloc&) {
+    return set_err(c, RX_ERR_NO_MEMORY, "rx_load_tree: host allocation failed");
+  }
 }
 
 int rx_load_legality(rx_ctx* c, uint32_t n_tris, const int32_t* tri_obj, const float* tri_normal) {
   if (!c || !tri_obj) return set_err(c, RX_ERR_INVALID, "rx_load_legality: null array");
-  c->tri_obj.assign(tri_obj, tri_obj + n_tris);
-  if (tri_normal)
-    c->tri_normal.assign(tri_normal, tri_normal + 3 * (size_t)n_tris);
-  else
-    c->tri_normal.clear();
-  if (c->tree_loaded) return upload_tree(c);
+  try {
+    c->tri_obj.assign(tri_obj, tri_obj + n_tris);
+    if (tri_normal)
+      c->tri_normal.assign(tri_normal, tri_normal + 3 * (size_t)n_tris);
+    else
+      c->tri_normal.clear();
+    if (c->tree_loaded) {
+      c->scene_loaded = false;  // the render scene's device tree is replaced
+      return upload_tree(c);
+    }
+  } catch (const std::bad_alloc&) {
+    c->tree_loaded = c->scene_loaded = false;
+    return set_err(c, RX_ERR_NO_MEMORY, "rx_load_legality: host allocation failed");
+  }
   return RX_OK;
 }
 
 int rx_intersect_dev(rx_ctx* c, uint32_t n, const float* org, const float* dir, const int32_t* inside,
-                     const int32_t* contact, uint32_t flags, float* dist, int32_t* tid, float* bary,
-                     uint32_t* counts) {
+                     const int32_t* contact, const int32_t* skip_tid, uint32_t flags, float* dist, int32_t* tid,
+                     float* bary, uint32_t* counts) {
   if (!c) return RX_ERR_INVALID;
   if (!c->tree_loaded) return set_err(c, RX_ERR_STATE, "rx_intersect: no tree loaded");
   if (n == 0) return RX_OK;
@@ -261,6 +289,7 @@ int rx_intersect_dev(rx_ctx* c, uint32_t n, const float* org, const float* dir, 
   a.dir = dir;
   a.inside = inside;
   a.contact = contact;
+  a.skip = skip_tid;
   a.dist = dist;
   a.tid = tid;
   a.bary = bary;
@@ -273,6 +302,7 @@ int rx_intersect_dev(rx_ctx* c, uint32_t n, const float* org, const float* dir, 
   if (int r = hip_err(c, hipEventRecord(c->ev0, c->stream), "event")) return r;
   if (int r = hip_err(c, launch_intersect(c->tree.dev, a, flags, blocks, c->stream), "intersect launch")) return r;
   if (int r = hip_err(c, hipEventRecord(c->ev1, c->stream), "event")) return r;
+  c->isect_queued = true;
   return RX_OK;
 }
 
@@ -283,8 +313,8 @@ int rx_last_kernel_ms(rx_ctx* c, float* ms) {
 }
 
 int rx_intersect(rx_ctx* c, uint32_t n, const float* org, const float* dir, const int32_t* inside,
-                 const int32_t* contact, uint32_t flags, float* dist, int32_t* tid, float* bary,
-                 uint32_t* counts) {
+                 const int32_t* contact, const int32_t* skip_tid, uint32_t flags, float* dist, int32_t* tid,
+                 float* bary, uint32_t* counts) {
   if (!c) return RX_ERR_INVALID;
   if (!c->tree_loaded) return set_err(c, RX_ERR_STATE, "rx_intersect: no tree loaded");
   if (n == 0) return RX_OK;
@@ -301,12 +331,14 @@ int rx_intersect(rx_ctx* c, uint32_t n, const float* org, const float* dir, cons
     RXCK(c->io_in.upload(inside, 4 * (size_t)n, s), "upload inside");
     RXCK(c->io_con.upload(contact, 4 * (size_t)n, s), "upload contact");
   }
+  if (skip_tid) RXCK(c->io_skip.upload(skip_tid, 4 * (size_t)n, s), "upload skip");
   RXCK(c->io_dist.ensure(4 * (size_t)n), "alloc dist");
   RXCK(c->io_tid.ensure(4 * (size_t)n), "alloc tid");
   if (bary) RXCK(c->io_bary.ensure(8 * (size_t)n), "alloc bary");
   if (counts) RXCK(c->io_counts.ensure(8 * (size_t)n), "alloc counts");
   int r = rx_intersect_dev(c, n, c->io_org.as<float>(), c->io_dir.as<float>(), leg ? c->io_in.as<int32_t>() : nullptr,
-                           leg ? c->io_con.as<int32_t>() : nullptr, flags, c->io_dist.as<float>(),
+                           leg ? c->io_con.as<int32_t>() : nullptr, skip_tid ? c->io_skip.as<int32_t>() : nullptr,
+                           flags, c->io_dist.as<float>(),
                            c->io_tid.as<int32_t>(), bary ? c->io_bary.as<float>() : nullptr,
                            counts ? c->io_counts.as<uint32_t>() : nullptr);
   if (r) return r;
@@ -317,6 +349,7 @@ int rx_intersect(rx_ctx* c, uint32_t n, const float* org, const float* dir, cons
   if (counts) RXCK(hipMemcpyAsync(counts, c->io_counts.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s), "download counts");
   RXCK(hipMemcpyAsync(&ovf, c->overflow.p, 4, hipMemcpyDeviceToHost, s), "download overflow");
   RXCK(hipStreamSynchronize(s), "synchronize");
+  c->isect_queued = false;  // its flag was read here
 #undef RXCK
   if (ovf) return set_err(c, RX_ERR_OVERFLOW, "rx_intersect: traversal stack overflow");
   return RX_OK;

@@ -11,6 +11,10 @@
 
 namespace rx {
 
+// rx_debug_fail_device_alloc: the countdown to a device allocation that is
+// made to fail (negative: off).
+extern int g_fail_alloc_countdown;
+
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -18,6 +22,7 @@ struct DBuf {
     if (n <= bytes && p) return hipSuccess;
     release();
     if (n == 0) n = 16;
+    if (g_fail_alloc_countdown >= 0 && g_fail_alloc_countdown-- == 0) return hipErrorOutOfMemory;
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
       p = nullptr;
@@ -77,7 +82,8 @@ struct rx_ctx {
   rx::TreeBufs tree;
   bool tree_loaded = false;
   // traversal scratch
-  rx::DBuf spill, overflow, io_org, io_dir, io_in, io_con, io_dist, io_tid, io_bary, io_counts;
+  rx::DBuf spill, overflow, io_org, io_dir, io_in, io_con, io_skip, io_dist, io_tid, io_bary, io_counts;
+  bool isect_queued = false;  // an rx_intersect_dev whose overflow flag rx_synchronize has not read yet
   float last_ms = 0.0f;
   // scene
   bool scene_loaded = false;

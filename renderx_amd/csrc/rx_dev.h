@@ -240,6 +240,7 @@ struct TraceState {
   bool anyhit_front;     // ... and with anyhit_front only a front-facing hit on an object other
   int32_t anyhit_skip;   // than anyhit_skip does
   int32_t bestLeaf, bestPos, ref, inside, contact;
+  int32_t skip;  // with SKIP: the triangle id the ray never hits (IntersectionGPU::Ray::last_tid), -1 none
   int sp;
 };
 
@@ -268,6 +269,7 @@ __device__ __forceinline__ bool trace_begin(const TreeRoot& R, TraceState& s, Ve
   s.use_anyhit = false;
   s.anyhit_front = false;
   s.anyhit_skip = INT32_MIN;
+  s.skip = -1;
   s.bestLeaf = s.bestPos = -1;
   s.sp = 0;
   s.ref = R.ref;
@@ -292,8 +294,10 @@ __device__ __forceinline__ bool trace_begin(const TreeRoot& R, TraceState& s, Ve
 // src/KDTree.cpp:317-321), inside a leaf the first triangle wins (:305).
 //
 // lds_stack: per-lane LDS column of LDS_DEPTH entries (stride = blockDim.x),
-// spill: per-thread global overflow of spill_depth entries.
-template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, bool ANYHIT = true>
+// spill: per-thread global overflow of spill_depth entries.  SKIP: the
+// triangle whose id is s.skip is passed over before its test, the GLSL
+// seam's last_tid rule (Shader/Intersection.frag:63-64).
+template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, bool ANYHIT = true, bool SKIP = false>
 __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint2* lds_stack, uint2* spill,
                                            int spill_depth, TraceStats* st, bool* overflow) {
   const float INF = __builtin_inff();
@@ -359,6 +363,7 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
     for (int32_t p = p0; p < p1; p++) {
       if (STATS) st->tris++;
       float4 a = T.tv0[p], b = T.tv1[p], c = T.tv2[p];
+      if (SKIP && __float_as_int(a.w) == s.skip) continue;
       float dist = tri_dist(xyz(a), xyz(b), xyz(c), s.o, s.d);
       if (!(dist >= 0)) continue;
       bool win = !s.has || dist < s.best || (dist == s.best && leaf > s.bestLeaf);
@@ -501,10 +506,11 @@ __device__ __forceinline__ uint32_t wave_take(WaveWork& w, unsigned long long id
 // Runs queue entries [0, n) over every wave of the grid (WaveWork): each lane
 // holds one ray and advances it a trip at a time, taking a new entry when its
 // ray has ended.  fetch(i, o, d, inside, contact, root) loads entry i (root
-// starts as T's own and may name another tree of the node array); emit(i,
-// hit, stats) stores its result.  A wave leaves once the queue is drained and
-// its lanes are idle.
-template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, class Fetch, class Emit>
+// starts as T's own and may name another tree of the node array; with SKIP a
+// seventh argument receives the triangle id the ray skips); emit(i, hit,
+// stats) stores its result.  A wave leaves once the queue is drained and its
+// lanes are idle.
+template <bool EXHAUSTIVE, bool LEGALITY, bool STATS, int LDS_DEPTH, bool SKIP = false, class Fetch, class Emit>
 __device__ __forceinline__ void trace_range(const DevTree& T, uint32_t n, uint32_t* work, uint2* lds_stack,
                                             uint2* spill, int spill_depth, bool* overflow, Fetch fetch, Emit emit) {
   bool active = false;
@@ -521,14 +527,18 @@ __device__ __forceinline__ void trace_range(const DevTree& T, uint32_t n, uint32
         Vec3 o, d;
         int32_t ins = -1, con = -1;
         TreeRoot root = tree_root(T);
-        fetch(i, o, d, ins, con, root);
+        int32_t skip = -1;
+        if constexpr (SKIP) fetch(i, o, d, ins, con, root, skip);
+        else fetch(i, o, d, ins, con, root);
         active = trace_begin<STATS>(root, ts, o, d, ins, con, &st);
+        ts.skip = skip;
         if (!active) emit(i, Hit{-1.0f, -1}, st);  // missed the scene box
       }
     }
     if (w.drained && !__any(active)) break;
     if (active &&
-        trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH, false>(T, ts, lds_stack, spill, spill_depth, &st, overflow)) {
+        trace_trip<EXHAUSTIVE, LEGALITY, STATS, LDS_DEPTH, false, SKIP>(T, ts, lds_stack, spill, spill_depth, &st,
+                                                                        overflow)) {
       emit(cur, ts.has ? Hit{ts.best, ts.bestPos} : Hit{-1.0f, -1}, st);
       active = false;
     }
@@ -541,12 +551,15 @@ __device__ __forceinline__ void trace_range(const DevTree& T, uint32_t n, uint32
 struct U4 {
   uint32_t x, y, z, w;
 };
-__device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
+  return (uint32_t)(((uint64_t)a * b) >> 32);  // v_mul_hi_u32 on the device
+}
+__host__ __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
   const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
   for (int i = 0; i < 10; i++) {
-    uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-    uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+    uint32_t hi0 = mulhi32(M0, c.x), lo0 = M0 * c.x;
+    uint32_t hi1 = mulhi32(M1, c.z), lo1 = M1 * c.z;
     c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
     k0 += W0;
     k1 += W1;
@@ -554,6 +567,6 @@ __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 // 24-bit uniform in [0, 1 - 2^-24], the reference's float form (src/RandGenerator.cpp:21-25)
-__device__ __forceinline__ float u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
+__host__ __device__ __forceinline__ float u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
 
 }  // namespace rx

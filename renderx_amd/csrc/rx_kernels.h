@@ -19,6 +19,7 @@ struct IsectArgs {
   const float* dir;
   const int32_t* inside;
   const int32_t* contact;
+  const int32_t* skip;  // per-ray triangle id never hit (IntersectionGPU::Ray::last_tid), or NULL
   float* dist;
   int32_t* tid;
   float* bary;

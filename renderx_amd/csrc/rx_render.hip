@@ -34,6 +34,12 @@ static void leaf_verts_of(const HostTree& t, const float* verts, const uint32_t*
 
 }  // namespace rx
 
+namespace rx {
+int load_scene_body(rx_ctx* c, uint32_t n_verts, const float* verts, uint32_t n_tris, const uint32_t* tris,
+                    uint32_t n_objects, const rx_object_desc* objects, uint32_t tree_flags,
+                    const std::vector<int32_t>& tri_obj, const std::vector<uint32_t>& local);
+}
+
 using namespace rx;
 
 extern "C" {
@@ -52,22 +58,41 @@ int rx_load_scene(rx_ctx* c, uint32_t n_verts, const float* verts, uint32_t n_tr
     return set_err(c, RX_ERR_INVALID, "rx_load_scene: empty scene");
   for (size_t i = 0; i < 3 * (size_t)n_tris; i++)
     if (tris[i] >= n_verts) return set_err(c, RX_ERR_INVALID, "rx_load_scene: vertex index out of range");
-  std::vector<int32_t> tri_obj(n_tris, -1);
-  std::vector<uint32_t> local(n_tris, 0);
-  for (uint32_t o = 0; o < n_objects; o++) {
-    const rx_object_desc& od = objects[o];
-    if (od.type < 0 || od.type > RX_OBJ_VPM) return set_err(c, RX_ERR_INVALID, "rx_load_scene: bad object type");
-    if ((uint64_t)od.tri_begin + od.tri_count > n_tris) return set_err(c, RX_ERR_INVALID, "rx_load_scene: object range");
-    for (uint32_t t = 0; t < od.tri_count; t++) {
-      if (tri_obj[od.tri_begin + t] != -1) return set_err(c, RX_ERR_INVALID, "rx_load_scene: overlapping objects");
-      tri_obj[od.tri_begin + t] = (int32_t)o;
-      local[od.tri_begin + t] = t;
+  try {
+    std::vector<int32_t> tri_obj(n_tris, -1);
+    std::vector<uint32_t> local(n_tris, 0);
+    for (uint32_t o = 0; o < n_objects; o++) {
+      const rx_object_desc& od = objects[o];
+      if (od.type < 0 || od.type > RX_OBJ_VPM) return set_err(c, RX_ERR_INVALID, "rx_load_scene: bad object type");
+      if ((uint64_t)od.tri_begin + od.tri_count > n_tris) return set_err(c, RX_ERR_INVALID, "rx_load_scene: object range");
+      for (uint32_t t = 0; t < od.tri_count; t++) {
+        if (tri_obj[od.tri_begin + t] != -1) return set_err(c, RX_ERR_INVALID, "rx_load_scene: overlapping objects");
+        tri_obj[od.tri_begin + t] = (int32_t)o;
+        local[od.tri_begin + t] = t;
+      }
     }
+    for (uint32_t t = 0; t < n_tris; t++)
+      if (tri_obj[t] < 0) return set_err(c, RX_ERR_INVALID, "rx_load_scene: triangle without object");
+    (void)hipSetDevice(c->device);
+    // from the first destructive step on, the context holds no scene and no
+    // tree until every upload below has succeeded
+    c->scene_loaded = false;
+    c->tree_loaded = false;
+    rx_render_state_free(c);
+    return load_scene_body(c, n_verts, verts, n_tris, tris, n_objects, objects, tree_flags, tri_obj, local);
+  } catch (const std::bad_alloc&) {
+    c->scene_loaded = c->tree_loaded = false;
+    return set_err(c, RX_ERR_NO_MEMORY, "rx_load_scene: host allocation failed");
   }
-  for (uint32_t t = 0; t < n_tris; t++)
-    if (tri_obj[t] < 0) return set_err(c, RX_ERR_INVALID, "rx_load_scene: triangle without object");
-  (void)hipSetDevice(c->device);
-  rx_render_state_free(c);
+}
+
+}  // extern "C"
+
+namespace rx {
+
+int load_scene_body(rx_ctx* c, uint32_t n_verts, const float* verts, uint32_t n_tris, const uint32_t* tris,
+                    uint32_t n_objects, const rx_object_desc* objects, uint32_t tree_flags,
+                    const std::vector<int32_t>& tri_obj, const std::vector<uint32_t>& local) {
   RenderState* R = new (std::nothrow) RenderState;
   if (!R) return RX_ERR_NO_MEMORY;
   c->render = R;
@@ -135,6 +160,10 @@ int rx_load_scene(rx_ctx* c, uint32_t n_verts, const float* verts, uint32_t n_tr
   c->scene_loaded = true;
   return RX_OK;
 }
+
+}  // namespace rx
+
+extern "C" {
 
 int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm, float* rgb_mean, double* sum,
               double* sumsq, rx_render_stats* stats) {
@@ -233,6 +262,9 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   uint32_t train_passes = 0, batch = 0, steps = 0, store_cap = 0;
   if (mode == RX_MODE_GUIDED) {
     if (!c->guide) return set_err(c, RX_ERR_STATE, "rx_render: guided mode needs rx_guide_init");
+    // a training pass keeps one record per pixel and bounce, ranked with 32-bit indices
+    if ((uint64_t)npix * (prm->max_depth + 1) >= (1ull << 31))
+      return set_err(c, RX_ERR_INVALID, "rx_render: guided mode needs pixels * (max_depth + 1) < 2^31");
     GuideState& G = *c->guide;
     for (int a = 0; a < 3; a++) {  // encode positions relative to the scene box (slightly padded)
       float pad = 1e-3f * (R.box_hi[a] - R.box_lo[a]) + 1e-6f;

@@ -1,6 +1,7 @@
 // Batch nearest-hit kernel: the HIP replacement of RenderX's GLSL traversal
 // (Shader/Intersection.frag:25-84, one fragment per ray) with the CPU
-// semantics of KDTree::intersect (src/KDTree.cpp:295-339).
+// semantics of KDTree::intersect (src/KDTree.cpp:295-339), plus the shader's
+// per-ray last_tid skip (Shader/Intersection.frag:63-64).
 //
 // Each wave takes chunks of the batch from shared counters, one ray per
 // lane, and lanes take the chunk's next ray as theirs ends (trace_range); the
@@ -15,15 +16,16 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(DevTree T, IsectArgs 
   __shared__ uint2 stack[kLdsDepth * kTraceBlock];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   bool ovf = false;
-  trace_range<EXH, LEG, STATS, kLdsDepth>(
+  trace_range<EXH, LEG, STATS, kLdsDepth, true>(
       T, a.n, a.work, stack + threadIdx.x, a.spill + (size_t)gtid * a.spill_depth, a.spill_depth, &ovf,
-      [&](uint32_t i, Vec3& o, Vec3& d, int32_t& ins, int32_t& con, TreeRoot&) {
+      [&](uint32_t i, Vec3& o, Vec3& d, int32_t& ins, int32_t& con, TreeRoot&, int32_t& skip) {
         o = v3(a.org[3 * (size_t)i], a.org[3 * (size_t)i + 1], a.org[3 * (size_t)i + 2]);
         d = v3(a.dir[3 * (size_t)i], a.dir[3 * (size_t)i + 1], a.dir[3 * (size_t)i + 2]);
         if (LEG) {
           ins = a.inside[i];
           con = a.contact[i];
         }
+        skip = a.skip ? a.skip[i] : -1;
       },
       [&](uint32_t i, Hit h, const TraceStats& st) {
         a.dist[i] = h.dist;

@@ -257,7 +257,16 @@ bool parse_serialized(uint32_t n_nodes, const float* nodes4, const float* min4, 
       if ((uint32_t)p.parent >= i) return false;
       depth[i] = depth[p.parent] + 1;
       if (depth[i] > out.max_depth) out.max_depth = depth[i];
+    } else if (i != 0) {
+      return false;  // only the root has no parent
     }
+  }
+  // the implicit links agree with the parent pointers (pre-order, left child
+  // = i + 1): the depths above are then the traversal's stack bound
+  for (uint32_t i = 0; i < n_nodes; i++) {
+    const PNode& p = out.nodes[i];
+    if (p.right < 0) continue;
+    if (out.nodes[i + 1].parent != (int32_t)i || out.nodes[p.right].parent != (int32_t)i) return false;
   }
   out.leaf_tris.resize(n_leaf);
   leaf_verts.resize(9 * (size_t)n_leaf);

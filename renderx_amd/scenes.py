@@ -193,6 +193,32 @@ def lit_plane(width=32, height=32, spp=64, max_depth=8, rr_start_depth=0) -> Sce
     return Scene("lit_plane", v, t, o, cam, spp, max_depth, rr_start_depth)
 
 
+def absorbing_slab(width=8, height=8, spp=64, max_depth=8, sigma_a=1.0, thickness=0.5, le=1.0) -> Scene:
+    """A purely absorbing, index-matched medium slab (sigma_s = 0, IOR 1) between
+    the camera and a large emitter that fills the view.  Every camera ray
+    crosses the slab's front and back faces and ends on the emitter, so a
+    pixel's expected value is le * exp(-sigma_a * l), l the length of its ray
+    inside the slab (meta["expected"], per pixel, from the pixel-centre ray).
+    A narrow field of view keeps l within 0.4 % of the thickness."""
+    b = _Builder()
+    sv, st = _axis_box([-0.8, -0.8, 0.0], [0.8, 0.8, thickness])
+    b.add(sv, st, VPM, (0.0, 0.0, 0.0), color2=(sigma_a, sigma_a, sigma_a), ior=1.0, g=0.0)
+    z = thickness + 1.5
+    ev, et = _quad([-1.5, -1.5, z], [1.5, -1.5, z], [1.5, 1.5, z], [-1.5, 1.5, z])
+    b.add(ev, _orient(ev, et, lambda c: np.tile([0, 0, -1.0], (len(c), 1))), EMISSIVE, (le, le, le))
+    v, t, o = b.finish()
+    cam = make_camera([0.0, 0.0, -3.0], [0.0, 0.0, 0.0], [0, 1, 0], width, height, fov_deg=10.0)
+    # pixel-centre ray directions (src/Camera.cpp:38-47 with the jitter at 1/2)
+    sd = float(cam["sight_dist"])
+    xs = np.arange(width) + 0.5 - width / 2
+    ys = height / 2 - (np.arange(height) + 0.5)
+    dx, dy = np.meshgrid(xs, ys)
+    cos_z = sd / np.sqrt(sd * sd + dx * dx + dy * dy)
+    expected = le * np.exp(-sigma_a * thickness / cos_z)
+    return Scene("absorbing_slab", v, t, o, cam, spp, max_depth, 0,
+                 meta={"expected": expected, "sigma_a": sigma_a, "thickness": thickness})
+
+
 # ---------------------------------------------------------------- geodesic spheres
 def icosphere(freq: int):
     """Class-I geodesic sphere: 20*freq^2 triangles, 10*freq^2+2 shared vertices, unit radius."""

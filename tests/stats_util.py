@@ -64,3 +64,13 @@ def three_sigma(s1, q1, n1, s2, q2, n2, pooled=True, mask=None):
 
 def rel_mse(img, ref, eps=1e-2):
     return float(np.mean((img - ref) ** 2 / (ref * ref + eps)))
+
+
+def closed_form_z(mean, sumsq_mean, spp, expected):
+    """Per pixel-channel z of a render's means against a closed-form image
+    expected (H, W): returns (max |z|, |z| of the image mean)."""
+    var = np.maximum(sumsq_mean - mean * mean, 0)
+    se = np.sqrt(var / spp) + 1e-9
+    z = (mean - expected[..., None]) / se
+    tot_se = np.sqrt((var / spp).sum()) / mean.size
+    return float(np.abs(z).max()), float(abs(mean.mean() - expected.mean()) / tot_se)

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from renderx_amd import scenes
-from tests.stats_util import clt_mask, three_sigma
+from tests.stats_util import closed_form_z, clt_mask, three_sigma
 
 pytestmark = pytest.mark.gpu
 
@@ -161,6 +161,22 @@ def test_furnace_sphere_closed_form(rx, gpu_ctx, mode):
     print(f"  f={f:.4f} expect {expect:.6f} got {got:.6f} (se {se:.2e})")
     # tessellation makes the sphere identity approximate: 1 % on top of the noise
     assert abs(got - expect) < 0.01 * expect + 5 * se
+
+
+@pytest.mark.parametrize("mode", [MODE_PT, MODE_NEE])
+def test_absorbing_slab_closed_form(rx, gpu_ctx, mode):
+    """A purely absorbing, index-matched medium slab in front of an emitter:
+    every pixel converges to Le exp(-sigma_a l) (Beer-Lambert), through the
+    medium's distance sampling, its P_surface origin density and the
+    transmittance of the segment (src/SceneVPMObject.cpp:4-251,
+    include/SceneVPMObject.h:27-47)."""
+    print(f"[absorbing slab] mode={mode}", flush=True)
+    sc = scenes.absorbing_slab(8, 8)
+    n = 65536
+    r = _render(rx, gpu_ctx, sc, n, mode)
+    zmax, zmean = closed_form_z(r["sum"] / n, r["sumsq"] / n, n, sc.meta["expected"])
+    print(f"  max |z| {zmax:.2f}, image-mean z {zmean:.2f}")
+    assert zmax < 5.0 and zmean < 4.0
 
 
 def test_launch_tuning_gives_identical_images(rx, gpu_ctx):

@@ -85,6 +85,37 @@ def test_legality_condition_bit_exact(rx, oracle_mod, gpu_ctx, name):
     assert np.array_equal(ge["tid"], e_tid)
 
 
+@pytest.mark.parametrize("name", ["cornell", "soup", "media"])
+def test_last_tid_skip_replays_testvisibility(rx, oracle_mod, gpu_ctx, name):
+    """The seam's per-ray skip (IntersectionGPU::Ray::last_tid,
+    Shader/Intersection.frag:63-64) on the pattern of Scene::testVisibility's
+    re-test (src/Scene.cpp:474-505): each ray re-cast from its first hit point
+    with last_tid = the hit triangle.  GPU ids and distances equal the oracle
+    recursion and a brute force with the same skip, bit for bit; no ray
+    returns its own triangle."""
+    print(f"[last_tid skip] case={name}", flush=True)
+    v, t, obj, o, d = _case(name)
+    ig = rx.IntersectionGPU(ctx=gpu_ctx)
+    ig.loadKDTreeToShader(rx.KDTree(v, t))
+    ot = oracle_mod.Tree(v, t, 0)
+    g0 = ig.query(o, d)
+    h = g0["tid"] >= 0
+    o2 = (o[h] + d[h] * g0["dist"][h][:, None]).astype(np.float32)
+    d2, last = d[h], g0["tid"][h]
+    e_dist, e_tid, _, _ = ot.intersect(o2, d2, 0, skip=last)
+    b_dist, b_tid, _, _ = ot.intersect(o2, d2, 2, skip=last)
+    assert np.array_equal(e_tid, b_tid) and _same_bits(e_dist, b_dist)
+    g = ig.query(o2, d2, last_tid=last)
+    assert np.array_equal(g["tid"], e_tid) and _same_bits(g["dist"], e_dist)
+    assert not np.any(g["tid"] == last)
+    ge = ig.query(o2, d2, last_tid=last, exhaustive=True)
+    assert np.array_equal(ge["tid"], e_tid) and _same_bits(ge["dist"], e_dist)
+    # -1 skips nothing: the same as no skip array
+    gn = ig.query(o, d, last_tid=np.full(len(o), -1, np.int32))
+    assert np.array_equal(gn["tid"], g0["tid"]) and _same_bits(gn["dist"], g0["dist"])
+    print(f"  {h.sum()} re-cast rays, {(g['tid'] >= 0).mean():.3f} hit a further surface")
+
+
 def test_load_tree_from_reference_serialization(rx, oracle_mod, gpu_ctx):
     """rx_load_tree takes KDTree::serializeForGPU's arrays unchanged (drop-in path)."""
     print("[drop-in serialized tree]", flush=True)
@@ -133,21 +164,30 @@ def test_work_distribution_covers_every_ray(rx, oracle_mod, gpu_ctx, n):
     assert np.array_equal(g["tid"], e_tid) and _same_bits(g["dist"], e_dist)
 
 
-def test_full_size_icosphere_2p22_rays(rx, oracle_mod, gpu_ctx):
-    """The 1,003,526-triangle scene and 2^22 rays: ordered == exhaustive on the
-    whole batch, and a 65,536-ray subset against the oracle recursion."""
-    print("[full size] icosphere 1M tris, 2^22 rays", flush=True)
-    sc = scenes.icosphere_scene(8, 8)
-    o, d = scenes.ray_batch(1.0, 1 << 22, 0x52454E44 + 1)
-    tree = rx.KDTree(sc.verts, sc.tris)
+def test_full_size_icosphere_2p22_rays(rx, gpu_ctx):
+    """The 1,003,526-triangle scene and all 2^22 rays of the hit-parity batch
+    against the oracle recursion: the GPU's (tid, dist bits) stream hashed per
+    2^16-ray chunk equals the committed digests of the oracle's stream
+    (tests/golden/icosphere_2p22_hits.json, made by
+    tests/golden/make_icosphere_hits.py), in ordered and exhaustive mode."""
+    import json
+    import os
+
+    from tests.golden import make_icosphere_hits as golden
+
+    print("[full size] icosphere 1M tris, 2^22 rays vs oracle digests", flush=True)
+    with open(os.path.join(os.path.dirname(golden.__file__), "icosphere_2p22_hits.json")) as f:
+        ref = json.load(f)
+    sc, o, d = golden.inputs()
+    assert golden.sha(o) == ref["input_sha256"]["org"] and golden.sha(d) == ref["input_sha256"]["dir"]
+    assert golden.sha(sc.verts) == ref["input_sha256"]["verts"] and golden.sha(sc.tris) == ref["input_sha256"]["tris"]
     ig = rx.IntersectionGPU(ctx=gpu_ctx)
-    ig.loadKDTreeToShader(tree)
+    ig.loadKDTreeToShader(rx.KDTree(sc.verts, sc.tris))
     g = ig.query(o, d)
     ms = ig.last_kernel_ms()
-    ge = ig.query(o, d, exhaustive=True)
-    assert np.array_equal(g["tid"], ge["tid"]) and _same_bits(g["dist"], ge["dist"])
     print(f"  ordered traversal {ms:.2f} ms = {len(o) / ms / 1e6:.3f} Grays/s, hit rate {(g['tid'] >= 0).mean():.3f}")
-    idx = np.random.default_rng(1).choice(len(o), 65536, replace=False)
-    ot = oracle_mod.Tree(sc.verts, sc.tris, 0)
-    e_dist, e_tid, _, _ = ot.intersect(o[idx], d[idx], 0)
-    assert np.array_equal(g["tid"][idx], e_tid) and _same_bits(g["dist"][idx], e_dist)
+    for mode, r in (("ordered", g), ("exhaustive", ig.query(o, d, exhaustive=True))):
+        dig = golden.chunk_digests(r["tid"], r["dist"])
+        bad = [k for k, (a, b) in enumerate(zip(dig, ref["chunk_sha256"])) if a != b]
+        assert len(dig) == len(ref["chunk_sha256"]) and not bad, f"{mode}: chunks differing from the oracle: {bad}"
+        assert int((r["tid"] >= 0).sum()) == ref["hits"]

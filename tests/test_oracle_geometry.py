@@ -175,3 +175,94 @@ def test_legality_condition_cases(oracle_mod):
     assert tid[1] == -1
     assert tid[2] in (2, 3) and dist[2] == 0.5
     assert tid[3] == -1
+
+
+@pytest.mark.parametrize("scale,shift", [(1e3, 0.0), (1e3, 5e3), (1e4, 1e5)])
+def test_ordered_cull_margin_at_large_coordinates(oracle_mod, scale, shift):
+    """The ordered traversal culls a subtree only when its slab entry exceeds
+    1.001 x the best distance (the GPU's rule, mirrored by the oracle's
+    ordered mode).  Secondary rays start on a surface, where hit distances are
+    short against the coordinates' magnitude; on a scene scaled and moved far
+    from the origin, with half of the rays near-grazing their start triangle,
+    the ordered winner must still be the reference recursion's, with the
+    legality condition and with the shader's last_tid skip."""
+    from renderx_amd import scenes
+
+    sc = scenes.icosphere_scene(8, 8, freq=24)
+    v = (sc.verts.astype(np.float64) * scale + shift).astype(np.float32)
+    tr = oracle_mod.Tree(v, sc.tris, 0)
+    rng = np.random.default_rng(1)
+    n = 40000
+    t = rng.integers(0, len(sc.tris), n)
+    P = v.astype(np.float64)[sc.tris[t]]
+    a, b = rng.random(n), rng.random(n)
+    m = a + b > 1
+    a[m], b[m] = 1 - a[m], 1 - b[m]
+    o = (P[:, 0] + a[:, None] * (P[:, 1] - P[:, 0]) + b[:, None] * (P[:, 2] - P[:, 0])).astype(np.float32)
+    nrm = oracle_mod.face_normals(v, sc.tris)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    nn = nrm[t].astype(np.float64)
+    dp = d - (d * nn).sum(1, keepdims=True) * nn
+    dp /= np.linalg.norm(dp, axis=1, keepdims=True)
+    dg = dp + (10 ** rng.uniform(-6, -2, n))[:, None] * nn * np.sign(rng.random(n) - 0.5)[:, None]
+    g = rng.random(n) < 0.5
+    d[g] = dg[g] / np.linalg.norm(dg[g], axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    obj = sc.tri_obj()
+    inside, contact = np.full(n, -1, np.int32), obj[t].astype(np.int32)
+    e = tr.intersect(o, d, 0, obj, nrm, inside, contact)
+    r = tr.intersect(o, d, 1, obj, nrm, inside, contact)
+    assert np.array_equal(e[1], r[1]) and np.array_equal(e[0].view(np.uint32), r[0].view(np.uint32))
+    assert (e[1] >= 0).mean() > 0.5
+    e = tr.intersect(o, d, 0, skip=t.astype(np.int32))
+    r = tr.intersect(o, d, 1, skip=t.astype(np.int32))
+    assert np.array_equal(e[1], r[1]) and np.array_equal(e[0].view(np.uint32), r[0].view(np.uint32))
+    assert not np.any(e[1] == t)  # the skipped triangle is never the hit
+
+
+def test_skip_tid_matches_brute_force(oracle_mod):
+    """IntersectionGPU::Ray::last_tid (Shader/Intersection.frag:63-64): the
+    named triangle is passed over; the recursion equals brute force with the
+    same skip, and rays re-cast from their hit point with the hit triangle
+    skipped (Scene::testVisibility's re-test, src/Scene.cpp:474-505) find the
+    next surface, never their own triangle at distance 0."""
+    from renderx_amd import scenes
+
+    sc = scenes.cornell(8, 8)
+    tr = oracle_mod.Tree(sc.verts, sc.tris, 0)
+    rng = np.random.default_rng(5)
+    o = rng.uniform(0.05, 0.95, (20000, 3)).astype(np.float32)
+    d = rng.normal(size=(20000, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    dist, tid, _, _ = tr.intersect(o, d, 0)
+    h = tid >= 0
+    o2 = (o[h] + d[h] * dist[h][:, None]).astype(np.float32)
+    e = tr.intersect(o2, d[h], 0, skip=tid[h])
+    b = tr.intersect(o2, d[h], 2, skip=tid[h])
+    assert np.array_equal(e[1], b[1]) and np.array_equal(e[0].view(np.uint32), b[0].view(np.uint32))
+    assert not np.any(e[1] == tid[h])
+    # without the skip most re-cast rays hit their own triangle again
+    z = tr.intersect(o2, d[h], 0)
+    assert (z[1] == tid[h]).mean() > 0.3
+
+
+def test_golden_icosphere_hits_reproduce(oracle_mod):
+    """The committed 2^22-ray digests (tests/golden/icosphere_2p22_hits.json)
+    match this checkout's scene and ray generator, and the oracle reproduces
+    the digests of four of the chunks (the GPU test compares all 64)."""
+    import json
+    import os
+
+    from tests.golden import make_icosphere_hits as golden
+
+    with open(os.path.join(os.path.dirname(golden.__file__), "icosphere_2p22_hits.json")) as f:
+        ref = json.load(f)
+    sc, o, d = golden.inputs()
+    assert golden.sha(o) == ref["input_sha256"]["org"] and golden.sha(d) == ref["input_sha256"]["dir"]
+    assert golden.sha(sc.verts) == ref["input_sha256"]["verts"] and golden.sha(sc.tris) == ref["input_sha256"]["tris"]
+    tree = oracle_mod.Tree(sc.verts, sc.tris, 0)
+    C = ref["chunk"]
+    for k in (0, 17, 40, 63):
+        dist, tid, _, _ = tree.intersect(o[k * C:(k + 1) * C], d[k * C:(k + 1) * C], 0)
+        assert golden.chunk_digests(tid, dist) == [ref["chunk_sha256"][k]]
