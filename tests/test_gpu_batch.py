@@ -68,3 +68,50 @@ def test_batch_rejects_mixed_modes_and_media(rx, gpu_ctx):
         br.add(_small(scenes.cornell(16, 16), 16, 16, 1), mode=MODE_NEE)
     with pytest.raises(rx.RxError):
         br.add(_small(scenes.media_scene(16, 16, freq=8, glass_freq=4), 16, 16, 1), mode=MODE_PT)
+
+
+# Eight scenes of the 64-scene workload at their true triangle counts (11 k to
+# 1.83 M, 4.0 M in all), at reduced resolution and spp
+REAL_SOUPS = [51, 26, 60, 18, 21, 2, 61, 13]
+
+
+def test_batch_of_real_soup_scenes(rx, oracle_mod, gpu_ctx):
+    """The batch renderer on scenes of the 64-scene workload at their true
+    sizes: every film bit-identical to the scene rendered alone, and the
+    smallest one within 3 sigma of the oracle's PathTracer."""
+    from tests.stats_util import clt_mask, three_sigma
+
+    print("[batch] 8 real soup scenes", flush=True)
+    scs = [_small(scenes.soup_scene(i), 24, 24, 4) for i in REAL_SOUPS]
+    print(f"  triangles: {[sc.n_tris for sc in scs]}", flush=True)
+    br = rx.BatchRenderer(ctx=gpu_ctx)
+    for k, sc in enumerate(scs):
+        br.add(sc, mode=MODE_PT, seed=500 + k)
+    try:
+        br.set_tuning(use_graph=1)
+        st = br.render()
+    finally:
+        br.set_tuning()
+    assert st["samples"] == sum(24 * 24 * 4 for _ in scs)
+    for k, sc in enumerate(scs):
+        a = rx.PathTracerHIP(sc, ctx=gpu_ctx).renderPixels(mode=MODE_PT, seed=500 + k)
+        img = br.image(k)
+        assert np.array_equal(img["sum"], a["sum"]) and np.array_equal(img["sumsq"], a["sumsq"]), k
+    print(f"  batch {st['ms_total']:.1f} ms, {st['rays']} rays; all films identical", flush=True)
+    # the smallest scene against the oracle
+    sc = _small(scenes.soup_scene(REAL_SOUPS[0]), 16, 16, 1)
+    n = 32768
+    g = rx.PathTracerHIP(sc, ctx=gpu_ctx).renderPixels(spp=n, mode=MODE_PT, seed=7)
+    osum, osq = oracle_mod.SceneOracle(sc).render(n, seed=8)
+    mask = clt_mask(g["sum"], g["sumsq"], osum, osq)
+    frac, zmean, _ = three_sigma(g["sum"], g["sumsq"], n, osum, osq, n, True, mask=mask)
+    print(f"  soup {REAL_SOUPS[0]} ({sc.n_tris} tris) vs oracle: within 3 sigma {frac:.4f}, mean z {zmean:+.3f}, "
+          f"excluded {1 - mask.mean():.4f}")
+    # a sparse soup lit by a small light: many pixels see the light on few
+    # paths, so their channels rest on too few effective samples for a z-test
+    # (clt_mask); the image mean, over every pixel, must agree as well
+    gm, om = g["sum"] / n, osum / n
+    se = np.sqrt((np.maximum(g["sumsq"] / n - gm * gm, 0).sum() + np.maximum(osq / n - om * om, 0).sum()) / n) / gm.size
+    print(f"  image mean gpu {gm.mean():.5f} oracle {om.mean():.5f} ({(gm.mean() - om.mean()) / se:+.2f} se)")
+    assert mask.mean() >= 0.85 and frac >= 0.995 and abs(zmean) < 0.1
+    assert abs(gm.mean() - om.mean()) < 4 * se

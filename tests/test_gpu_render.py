@@ -59,6 +59,9 @@ CASES = {
     "media": (lambda: scenes.media_scene(RES, RES, spp=16, max_depth=16, freq=24, glass_freq=8), 32768),
     # a grey scene: the three channels of a pixel fail together, so more pixels
     "icosphere": (lambda: scenes.icosphere_scene(32, 32, spp=16, max_depth=10, freq=24), 32768),
+    # the participating-media workload at its real geometry (499,280-triangle
+    # displaced sphere, 5,120-triangle glass sphere), at reduced resolution
+    "media_full": (lambda: scenes.media_scene(16, 16, spp=16, max_depth=16, freq=158, glass_freq=16), 16384),
 }
 _oracle_cache = {}
 _gpu_pt_cache = {}

@@ -30,6 +30,10 @@ def _case(name):
         s = scenes.soup_scene(2, n_tris=40000)
         o, d = _rays(50000, 3, -0.2, 1.2)
         return s.verts, s.tris, s.tri_obj(), o, d
+    if name == "media_full":  # the participating-media workload's real geometry (504,448 triangles)
+        s = scenes.media_scene(8, 8, freq=158, glass_freq=16)
+        o, d = _rays(200000, 5, 0.0, 1.0)
+        return s.verts, s.tris, s.tri_obj(), o, d
     s = scenes.media_scene(8, 8, freq=40, glass_freq=8)
     o, d = _rays(50000, 4, 0.0, 1.0)
     return s.verts, s.tris, s.tri_obj(), o, d
@@ -39,7 +43,7 @@ def _same_bits(a, b):
     return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
 
 
-@pytest.mark.parametrize("name", ["cornell", "sphere", "soup", "media"])
+@pytest.mark.parametrize("name", ["cornell", "sphere", "soup", "media", "media_full"])
 def test_hits_bit_exact_and_counts(rx, oracle_mod, gpu_ctx, name):
     print(f"[traversal parity] case={name}", flush=True)
     v, t, obj, o, d = _case(name)
@@ -63,7 +67,7 @@ def test_hits_bit_exact_and_counts(rx, oracle_mod, gpu_ctx, name):
     print(f"  hits={hit.mean():.3f} boxes/ray ordered={r_box.mean():.1f} exhaustive={e_box.mean():.1f}")
 
 
-@pytest.mark.parametrize("name", ["cornell", "media"])
+@pytest.mark.parametrize("name", ["cornell", "media", "media_full"])
 def test_legality_condition_bit_exact(rx, oracle_mod, gpu_ctx, name):
     print(f"[legality parity] case={name}", flush=True)
     v, t, obj, o, d = _case(name)
