@@ -197,6 +197,20 @@ int rx_intersect(rx_ctx* ctx, uint32_t n, const float* org, const float* dir, co
 int rx_intersect_dev(rx_ctx* ctx, uint32_t n, const float* org, const float* dir, const int32_t* inside,
                      const int32_t* contact, const int32_t* skip_tid, uint32_t flags, float* dist, int32_t* tid,
                      float* bary, uint32_t* counts);
+/* No counterpart in the reference: rx_intersect_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; the
+ * traversal-stack spill and the work counters live in a fixed 32 MiB device
+ * array of the library (one per GPU, held until the process ends) instead of
+ * buffers the context allocates, and they are cleared by a kernel, not a
+ * memset.  Trees whose depth needs more spill than that array holds for one
+ * 128-thread block are rejected with RX_ERR_INVALID.  Calls that overlap on
+ * one GPU (any context, any stream) share the array and race: order them on
+ * one stream or synchronize between them.  A stack overflow is reported by
+ * the next rx_synchronize; rx_last_kernel_ms does not time these calls. */
+int rx_intersect_dev_noalloc(rx_ctx* ctx, uint32_t n, const float* org, const float* dir, const int32_t* inside,
+                             const int32_t* contact, const int32_t* skip_tid, uint32_t flags, float* dist,
+                             int32_t* tid, float* bary, uint32_t* counts);
 /* Duration in ms of the last rx_intersect(_dev) traversal kernel (HIP events
  * on the context stream; valid after rx_synchronize). */
 int rx_last_kernel_ms(rx_ctx* ctx, float* ms);

@@ -35,6 +35,7 @@ SIGNATURES = {
     "rx_load_legality": (C.c_int, [P, U32, P, P]),
     "rx_intersect": (C.c_int, [P, U32, P, P, P, P, P, U32, P, P, P, P]),
     "rx_intersect_dev": (C.c_int, [P, U32, P, P, P, P, P, U32, P, P, P, P]),
+    "rx_intersect_dev_noalloc": (C.c_int, [P, U32, P, P, P, P, P, U32, P, P, P, P]),
     "rx_last_kernel_ms": (C.c_int, [P, C.POINTER(F32)]),
     "rx_copy_kernel": (C.c_int, [P, P, P, U64]),
     "rx_load_scene": (C.c_int, [P, U32, P, U32, P, U32, P, U32]),

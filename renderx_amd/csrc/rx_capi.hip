@@ -2,6 +2,7 @@
 // Scene loading and rendering live in rx_render.hip.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "rx_context.h"
@@ -16,6 +17,38 @@ struct rx_tree {
 namespace rx {
 
 int g_fail_alloc_countdown = -1;
+
+// Traversal scratch of rx_intersect_dev_noalloc: its work counters and
+// overflow flag, then its traversal-stack spill, in a file-scope device array
+// instead of per-call allocations.  Each GPU the library runs on holds its
+// own copy until the process ends.  Host code never names the array in a
+// kernel argument or copy: it uses the per-device address from
+// hipGetSymbolAddress (scratch_addr).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr size_t kScratchHeader = 4096;  // overflow flag + work counters, then the spill
+static_assert(4 * (1 + kWorkWords) <= kScratchHeader, "work counters fit the scratch header");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kMaxDevices];
+
+// The device address of `scratch` on `device` (current), looked up once per device.
+static int scratch_addr(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return RX_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return RX_ERR_HIP;
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return RX_OK;
+}
+
+__global__ void k_zero_words(uint32_t* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
 
 int set_err(rx_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -194,6 +227,10 @@ int rx_create(int device, rx_ctx** out) {
     return RX_ERR_NO_DEVICE;
   }
   c->n_cu = prop.multiProcessorCount;
+  if (scratch_addr(device, &c->scratch) != RX_OK) {  // the no-allocation entry point's scratch on this device
+    rx_destroy(c);
+    return RX_ERR_HIP;
+  }
   *out = c;
   return RX_OK;
 }
@@ -223,10 +260,13 @@ int rx_debug_fail_device_alloc(int32_t nth) {
 int rx_synchronize(rx_ctx* c) {
   if (!c) return RX_ERR_INVALID;
   if (int r = hip_err(c, hipStreamSynchronize(c->stream), "synchronize")) return r;
-  if (c->isect_queued && c->overflow.p) {  // the traversal stack flag of the queued rx_intersect_dev calls
-    c->isect_queued = false;
+  // the traversal stack flags of the queued rx_intersect_dev / rx_intersect_dev_noalloc calls
+  const void* flags[2] = {c->isect_queued ? c->overflow.p : nullptr, c->noalloc_queued ? c->scratch : nullptr};
+  c->isect_queued = c->noalloc_queued = false;
+  for (const void* f : flags) {
+    if (!f) continue;
     int32_t ovf = 0;
-    if (int r = hip_err(c, hipMemcpy(&ovf, c->overflow.p, 4, hipMemcpyDeviceToHost), "overflow flag")) return r;
+    if (int r = hip_err(c, hipMemcpy(&ovf, f, 4, hipMemcpyDeviceToHost), "overflow flag")) return r;
     if (ovf) return set_err(c, RX_ERR_OVERFLOW, "rx_intersect_dev: traversal stack overflow");
   }
   return RX_OK;
@@ -303,6 +343,54 @@ int rx_intersect_dev(rx_ctx* c, uint32_t n, const float* org, const float* dir, 
   if (int r = hip_err(c, launch_intersect(c->tree.dev, a, flags, blocks, c->stream), "intersect launch")) return r;
   if (int r = hip_err(c, hipEventRecord(c->ev1, c->stream), "event")) return r;
   c->isect_queued = true;
+  return RX_OK;
+}
+
+int rx_intersect_dev_noalloc(rx_ctx* c, uint32_t n, const float* org, const float* dir, const int32_t* inside,
+                             const int32_t* contact, const int32_t* skip_tid, uint32_t flags, float* dist,
+                             int32_t* tid, float* bary, uint32_t* counts) {
+  if (!c) return RX_ERR_INVALID;
+  if (!c->tree_loaded) return set_err(c, RX_ERR_STATE, "rx_intersect_dev_noalloc: no tree loaded");
+  if (!c->scratch) return set_err(c, RX_ERR_STATE, "rx_intersect_dev_noalloc: no scratch on this device");
+  if (n == 0) return RX_OK;
+  if (!org || !dir || !dist || !tid) return set_err(c, RX_ERR_INVALID, "rx_intersect_dev_noalloc: null array");
+  if ((flags & RX_ISECT_LEGALITY) && (!inside || !contact))
+    return set_err(c, RX_ERR_INVALID, "rx_intersect_dev_noalloc: legality needs inside/contact");
+  (void)hipSetDevice(c->device);
+  const int block = kTraceBlock;
+  // the spill each thread needs beyond the LDS stack, and how many threads' worth the scratch holds
+  const int sd = std::max(1, (int)c->tree.max_depth + 2 - kLdsDepth);
+  const size_t per_block = (size_t)block * sd * sizeof(uint2);
+  const size_t fit = (kScratchBytes - kScratchHeader) / per_block;
+  if (fit < 1)
+    return set_err(c, RX_ERR_INVALID,
+                   "rx_intersect_dev_noalloc: the tree is too deep for the fixed traversal scratch (" +
+                       std::to_string(per_block) + " bytes per block needed, " +
+                       std::to_string(kScratchBytes - kScratchHeader) + " available)");
+  const int blocks = (int)std::min<uint64_t>(std::min<uint64_t>((n + block - 1) / block, (uint64_t)max_blocks(c, block)),
+                                             (uint64_t)fit);
+  unsigned char* base = static_cast<unsigned char*>(c->scratch);
+  IsectArgs a;
+  a.n = n;
+  a.org = org;
+  a.dir = dir;
+  a.inside = inside;
+  a.contact = contact;
+  a.skip = skip_tid;
+  a.dist = dist;
+  a.tid = tid;
+  a.bary = bary;
+  a.counts = counts;
+  a.overflow = reinterpret_cast<int32_t*>(base);
+  a.work = reinterpret_cast<uint32_t*>(base) + 1;
+  a.spill = reinterpret_cast<uint2*>(base + kScratchHeader);
+  a.spill_depth = sd;
+  // the previous call's counters are still there: clear them with a kernel (no memset node under capture)
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, c->stream, reinterpret_cast<uint32_t*>(base),
+                     (uint32_t)(1 + kWorkWords));
+  if (int r = hip_err(c, hipGetLastError(), "scratch clear")) return r;
+  if (int r = hip_err(c, launch_intersect(c->tree.dev, a, flags, blocks, c->stream), "intersect launch")) return r;
+  c->noalloc_queued = true;
   return RX_OK;
 }
 

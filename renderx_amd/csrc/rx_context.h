@@ -83,7 +83,9 @@ struct rx_ctx {
   bool tree_loaded = false;
   // traversal scratch
   rx::DBuf spill, overflow, io_org, io_dir, io_in, io_con, io_skip, io_dist, io_tid, io_bary, io_counts;
-  bool isect_queued = false;  // an rx_intersect_dev whose overflow flag rx_synchronize has not read yet
+  bool isect_queued = false;    // an rx_intersect_dev whose overflow flag rx_synchronize has not read yet
+  bool noalloc_queued = false;  // the same for rx_intersect_dev_noalloc (flag at the start of `scratch`)
+  void* scratch = nullptr;      // this device's address of rx_intersect_dev_noalloc's static scratch
   float last_ms = 0.0f;
   // scene
   bool scene_loaded = false;
