@@ -24,14 +24,21 @@ def _render_rc(rx, ctx, sc):
     return lib().rx_render(ctx.h, ptr(cam), ptr(p), ptr(mean), None, None, None)
 
 
-def _intersect_rc(rx, ctx):
+def _intersect_rc(rx, ctx, want=None):
+    """rx_intersect's status on a few rays through the scene; with `want`
+    (tid array) a successful call must also return those hits."""
     from renderx_amd._lib import lib, ptr
 
-    o = np.full((4, 3), 0.5, np.float32)
-    d = np.tile(np.array([0, 0, 1], np.float32), (4, 1))
-    dist = np.empty(4, np.float32)
-    tid = np.empty(4, np.int32)
-    return lib().rx_intersect(ctx.h, 4, ptr(o), ptr(d), None, None, None, 0, ptr(dist), ptr(tid), None, None)
+    rng = np.random.default_rng(2)
+    o = rng.uniform(0.1, 0.9, (256, 3)).astype(np.float32)
+    d = rng.normal(size=(256, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    dist = np.empty(256, np.float32)
+    tid = np.empty(256, np.int32)
+    rc = lib().rx_intersect(ctx.h, 256, ptr(o), ptr(d), None, None, None, 0, ptr(dist), ptr(tid), None, None)
+    if rc == 0 and want is not None:
+        assert np.array_equal(tid, want)
+    return rc, tid
 
 
 def test_interrupted_loads_leave_no_half_built_state(rx):
@@ -40,14 +47,19 @@ def test_interrupted_loads_leave_no_half_built_state(rx):
     print("[error paths] failed device allocations during rx_load_scene", flush=True)
     ctx = rx.Context(0)
     sc = scenes.media_scene(16, 16, freq=8, glass_freq=4)  # global tree + a medium tree + objects + emitters
-    ref = rx.PathTracerHIP(sc, ctx=rx.Context(0)).renderPixels(spp=16, mode=rx.MODE_NEE, seed=3)["sum"]
-    v = np.ascontiguousarray(sc.verts, np.float32)
-    t = np.ascontiguousarray(sc.tris, np.uint32)
-    o = np.ascontiguousarray(sc.objects)
-    failures = 0
+    good = rx.Context(0)
+    ref = rx.PathTracerHIP(sc, ctx=good).renderPixels(spp=16, mode=rx.MODE_NEE, seed=3)["sum"]
+    _, want = _intersect_rc(rx, good)
+    small = scenes.cornell(8, 8)
+    arr = lambda s: (np.ascontiguousarray(s.verts, np.float32), np.ascontiguousarray(s.tris, np.uint32),
+                     np.ascontiguousarray(s.objects))
+    v, t, o = arr(sc)
+    sv, st, so = arr(small)
+    failures, tree_kept = 0, 0
     try:
         for k in range(64):  # make the k-th device allocation of the load fail, until a load gets through
-            assert lib().rx_load_scene(ctx.h, len(v), ptr(v), len(t), ptr(t), len(o), ptr(o), 0) == 0
+            # a smaller scene first, so that every buffer of the big load must grow
+            assert lib().rx_load_scene(ctx.h, len(sv), ptr(sv), len(st), ptr(st), len(so), ptr(so), 0) == 0
             lib().rx_debug_fail_device_alloc(k)
             rc = lib().rx_load_scene(ctx.h, len(v), ptr(v), len(t), ptr(t), len(o), ptr(o), 0)
             lib().rx_debug_fail_device_alloc(-1)
@@ -56,11 +68,14 @@ def test_interrupted_loads_leave_no_half_built_state(rx):
             failures += 1
             assert rc == RX_ERR_NO_MEMORY, (k, rc)
             assert _render_rc(rx, ctx, sc) == RX_ERR_STATE, k
-            assert _intersect_rc(rx, ctx) == RX_ERR_STATE, k
+            # the global tree is either absent or whole (a later stage failed): never half-built
+            irc, _ = _intersect_rc(rx, ctx, want)
+            assert irc in (0, RX_ERR_STATE), (k, irc)
+            tree_kept += irc == 0
     finally:
         lib().rx_debug_fail_device_alloc(-1)
-    print(f"  {failures} interrupted loads, each left RX_ERR_STATE")
-    assert failures >= 3
+    print(f"  {failures} interrupted loads ({tree_kept} after the global tree was complete), each without a scene")
+    assert failures >= 5 and 0 < tree_kept < failures
     pt = rx.PathTracerHIP(sc, ctx=ctx)
     assert np.array_equal(pt.renderPixels(spp=16, mode=rx.MODE_NEE, seed=3)["sum"], ref)
 
@@ -74,7 +89,7 @@ def test_malformed_tree_is_rejected_and_clears_the_old_one(rx, oracle_mod):
     arrs = [np.ascontiguousarray(a, np.float32) for a in oracle_mod.Tree(sc.verts, sc.tris, 0).serialize()]
     ig = rx.IntersectionGPU(ctx=ctx)
     ig.loadKDTreeToShader(arrs)
-    assert _intersect_rc(rx, ctx) == 0
+    assert _intersect_rc(rx, ctx)[0] == 0
     nodes = arrs[0].reshape(-1, 4).copy()
     interior = np.nonzero(nodes[:, 1] >= 0)[0]
     bad = nodes.copy()
@@ -82,9 +97,9 @@ def test_malformed_tree_is_rejected_and_clears_the_old_one(rx, oracle_mod):
     rc = lib().rx_load_tree(ctx.h, len(bad), ptr(bad), ptr(arrs[1]), ptr(arrs[2]), len(arrs[3]),
                             ptr(arrs[3]), ptr(arrs[4]), ptr(arrs[5]))
     assert rc == RX_ERR_INVALID
-    assert _intersect_rc(rx, ctx) == RX_ERR_STATE
+    assert _intersect_rc(rx, ctx)[0] == RX_ERR_STATE
     ig.loadKDTreeToShader(arrs)
-    assert _intersect_rc(rx, ctx) == 0
+    assert _intersect_rc(rx, ctx)[0] == 0
 
 
 def test_guided_record_index_limit_is_checked(rx):
