@@ -321,8 +321,13 @@ typedef struct rx_guide_params {
   float learning_rate;     /* Adam */
   float beta1, beta2, eps;
   uint32_t seed;
-  uint32_t flags;
+  uint32_t flags;          /* rx_guide_flags */
 } rx_guide_params;
+
+enum rx_guide_flags {
+  RX_GUIDE_LR_DECAY = 1    /* the Adam learning rate falls geometrically from learning_rate to a tenth of it
+                              over the online warm-up's train_passes * steps_per_pass steps */
+};
 
 /* The guiding network has no counterpart in the reference.  It maps a
  * frequency encoding of (position normalised to the scene box: 12 sin/cos

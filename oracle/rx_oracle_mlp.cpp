@@ -9,7 +9,7 @@
 //   lobe j (0..7): z[5j] weight logit, z[5j+1..5j+3] mean direction (unnormalised),
 //                  z[5j+4] log concentration;  z[40] selection logit.
 //   pi = softmax(logits), mu_j = m_j/|m_j|, kappa_j = exp(clamp(lk_j, -3, 8)),
-//   alpha = 0.05 + 0.9 sigmoid(z[40]).
+//   alpha = 0.05 + 0.45 sigmoid(z[40]) (at most half of the samples guided).
 //   vMF_j(w) = kappa/(2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1))
 //   q(w) = sum_j pi_j vMF_j(w);  p_mix = alpha q + (1 - alpha) p_bsdf
 //   loss = -weight * log p_mix(w)
@@ -47,7 +47,7 @@ void head_of(const double* z, Head& h) {
     h.kappa[j] = std::exp(lk);
   }
   h.sig = 1.0 / (1.0 + std::exp(-z[40]));
-  h.alpha = 0.05 + 0.9 * h.sig;
+  h.alpha = 0.05 + 0.45 * h.sig;
 }
 
 double vmf(double kappa, const double* mu, const double* w) {
@@ -146,7 +146,7 @@ double orc_mlp_loss(int n, const double* z4, const float* w, const float* pb, co
     for (int c = 0; c < OUT; c++) g[c] = 0;
     double gp = -wt[r] / pm;
     double dq = gp * h.alpha;
-    g[40] = gp * (q - pb[r]) * 0.9 * h.sig * (1 - h.sig);
+    g[40] = gp * (q - pb[r]) * 0.45 * h.sig * (1 - h.sig);
     for (int j = 0; j < NL; j++) {
       g[5 * j] = dq * h.pi[j] * (v[j] - q);
       double dv = dq * h.pi[j];

@@ -30,6 +30,7 @@ RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use
 GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("batch_size", "<u4"), ("steps_per_pass", "<u4"), ("learning_rate", "<f4"),
                          ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4")])
 MODE_PT, MODE_NEE, MODE_GUIDED, MODE_BPT = 0, 1, 2, 3
+GUIDE_LR_DECAY = 1
 
 
 class KDTree:
@@ -139,8 +140,10 @@ class PathTracerHIP:
         self.ctx.check(lib().rx_load_scene(self.ctx.h, len(v), ptr(v), len(t), ptr(t), len(o), ptr(o),
                                            1 if tree_abs_int else 0), "rx_load_scene")
 
-    def guide_init(self, train_passes=16, batch_size=4096, steps_per_pass=16, learning_rate=1e-2, seed=7):
+    def guide_init(self, train_passes=16, batch_size=4096, steps_per_pass=16, learning_rate=1e-2, seed=7, flags=0):
+        """rx_guide_init: the network and its online warm-up (flags: GUIDE_LR_DECAY)."""
         p = np.zeros(1, GUIDE_PARAMS)
+        p["flags"] = flags
         p["train_passes"] = train_passes
         p["batch_size"] = batch_size
         p["steps_per_pass"] = steps_per_pass

@@ -10,6 +10,12 @@
 namespace rx {
 
 constexpr int kGIn = 96, kGHid = 64, kGOut = 48, kGLobes = 8;
+// Selection probability of the learned proposal: alpha = kAlphaMin +
+// kAlphaSpan sigmoid(z[40]) in [0.05, 0.5].  Keeping at least half of every
+// guided vertex's samples on the BSDF / phase function bounds the vertex's
+// weight f / p_mix by twice the plain strategy's, so throughputs cannot grow
+// geometrically along a path whose directions the mixture misses.
+constexpr float kAlphaMin = 0.05f, kAlphaSpan = 0.45f;
 constexpr int kGParams = kGIn * kGHid + 2 * kGHid * kGHid + kGHid * kGOut + 3 * kGHid + kGOut;  // 17,648
 
 struct GuideNet {
@@ -19,47 +25,56 @@ struct GuideNet {
   float lo[3], hi[3];     // encoding bounds (scene box)
 };
 
-// Head of one row of the network output z (48 floats): the softmax shift and
-// scale of the mixture weights and the selection probability alpha are kept,
-// each lobe (weight, unit mean, concentration) is decoded where it is used,
-// so a renderer thread holds 4 values instead of the 49 of a decoded head.
+// Decoded network output of one renderer query, kGHead bytes: per lobe j a
+// 16-byte record (unit mean mu_j as 3 fp16, concentration kappa_j as fp16,
+// mixture weight pi_j as fp32, vMF normaliser kappa_j / (2 pi (1 - e^{-2
+// kappa_j})) of the stored kappa_j as fp32), then the selection probability
+// alpha (fp32).  The sampler and the density read the same stored values:
+// weights divided by their stored sum, means renormalised, so the density is
+// exactly the sampler's.  Approximate reciprocals and square roots are used
+// on purpose: both sides use the same ones.
+typedef _Float16 half_t;
+struct __attribute__((aligned(16))) GuideLobe {
+  half_t mu[3], kappa;
+  float pi, norm;
+};
+constexpr int kGHead = 8 * (int)sizeof(GuideLobe) + 16;  // 144 bytes per row
+static_assert(sizeof(GuideLobe) == 16, "lobe record is 16 bytes");
+
 struct GuideHead {
-  const float* z;
-  float mx, inv, alpha;
+  const GuideLobe* l;
+  float inv, alpha;
 };
 
-__device__ __forceinline__ void guide_head(const float* z, GuideHead& h) {
-  float mx = z[0];
-  for (int j = 1; j < kGLobes; j++) mx = fmaxf(mx, z[5 * j]);
+__device__ __forceinline__ void guide_head(const unsigned char* row, GuideHead& g) {
+  g.l = reinterpret_cast<const GuideLobe*>(row);
   float s = 0.0f;
-  for (int j = 0; j < kGLobes; j++) s += __expf(z[5 * j] - mx);
-  h.z = z;
-  h.mx = mx;
-  h.inv = 1.0f / s;
-  h.alpha = 0.05f + 0.9f / (1.0f + __expf(-z[40]));
+  for (int j = 0; j < kGLobes; j++) s += g.l[j].pi;
+  g.inv = __builtin_amdgcn_rcpf(s);
+  g.alpha = *reinterpret_cast<const float*>(row + 8 * sizeof(GuideLobe));
 }
-__device__ __forceinline__ float guide_weight(const GuideHead& h, int j) { return __expf(h.z[5 * j] - h.mx) * h.inv; }
-__device__ __forceinline__ void guide_lobe(const GuideHead& h, int j, float mu[3], float& kappa) {
-  const float* z = h.z + 5 * j;
-  float m0 = z[1], m1 = z[2], m2 = z[3];
-  float l = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
-  mu[0] = m0 / l;
-  mu[1] = m1 / l;
-  mu[2] = m2 / l;
-  float lk = fminf(8.0f, fmaxf(-3.0f, z[4]));
-  kappa = __expf(lk);
+__device__ __forceinline__ float guide_weight(const GuideHead& g, int j) { return g.l[j].pi * g.inv; }
+// Lobe j: unit mean, concentration and normaliser.
+__device__ __forceinline__ void guide_lobe(const GuideHead& g, int j, Vec3& mu, float& kappa, float& norm) {
+  const GuideLobe L = g.l[j];
+  const float m0 = (float)L.mu[0], m1 = (float)L.mu[1], m2 = (float)L.mu[2];
+  const float r = __builtin_amdgcn_rsqf(fmaxf(m0 * m0 + m1 * m1 + m2 * m2, 1e-12f));
+  mu = v3(m0 * r, m1 * r, m2 * r);
+  kappa = (float)L.kappa;
+  norm = L.norm;
 }
 
 __device__ __forceinline__ float guide_pdf(const GuideHead& h, Vec3 w) {
   float q = 0.0f;
+#pragma unroll
   for (int j = 0; j < kGLobes; j++) {
-    float mu[3], k;
-    guide_lobe(h, j, mu, k);
-    const float norm = k / (6.28318530717958647692f * (1.0f - __expf(-2.0f * k)));  // kappa / (2 pi (1 - e^-2kappa))
-    float c = mu[0] * w.x + mu[1] * w.y + mu[2] * w.z;
-    q += guide_weight(h, j) * norm * __expf(k * (c - 1.0f));
+    Vec3 mu;
+    float k, norm;
+    guide_lobe(h, j, mu, k, norm);
+    const float c = mu.x * w.x + mu.y * w.y + mu.z * w.z;
+    q += h.l[j].pi * norm * __expf(k * (c - 1.0f));
   }
-  return q;
+  return q * h.inv;
 }
 
 // Lobe pick by u0, then the numerically stable vMF sampler (W. Jakob,
@@ -71,14 +86,14 @@ __device__ __forceinline__ Vec3 guide_sample(const GuideHead& h, float u0, float
     j++;
     c += guide_weight(h, j);
   }
-  float mu[3], k;
-  guide_lobe(h, j, mu, k);
-  float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * k)) / k;
+  Vec3 m;
+  float k, norm;
+  guide_lobe(h, j, m, k, norm);
+  float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * k)) * __builtin_amdgcn_rcpf(k);
   w = fminf(1.0f, fmaxf(-1.0f, w));
   float r = sqrtf(fmaxf(0.0f, 1.0f - w * w));
   float sp, cp;
   __sincosf(6.28318530717958647692f * u2, &sp, &cp);
-  Vec3 m = v3(mu[0], mu[1], mu[2]);
   Vec3 tmp = fabsf(m.x) > 0.9f ? v3(0, 1, 0) : v3(1, 0, 0);
   Vec3 t1 = normalize(cross(m, tmp));
   Vec3 t2 = cross(m, t1);
@@ -122,8 +137,77 @@ __device__ __forceinline__ void guide_encode(const GuideNet& N, Vec3 p, Vec3 d, 
   for (int v = 0; v < kGIn / 8; v++) o4[v] = make_uint4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
 }
 
+// One (sin, cos) pair of the encoding: position octave k of axis a
+// (pair 3k + a, k < 12) or direction octave k (pair 36 + 3k + a, k < 4),
+// the same values guide_encode writes.
+__device__ __forceinline__ uint32_t guide_pair(const GuideNet& N, Vec3 p, Vec3 d, int pair) {
+  const float pc[3] = {p.x, p.y, p.z}, dc[3] = {d.x, d.y, d.z};
+  float f;
+  if (pair < 36) {
+    const int k = pair / 3, a = pair % 3;
+    float x = (pc[a] - N.lo[a]) / (N.hi[a] - N.lo[a]);
+    x = fminf(1.0f, fmaxf(0.0f, x));
+    f = ldexpf(x, k);  // x doubled k times, exactly
+  } else {
+    const int k = (pair - 36) / 3, a = (pair - 36) % 3;
+    f = ldexpf(dc[a], k);
+  }
+  float sn, cs;
+  sincospif(f, &sn, &cs);
+  return (uint32_t)f2bf(sn) | ((uint32_t)f2bf(cs) << 16);
+}
+
+// ------------------------------------------------------------------ MFMA tiles
+// Every product of the network has one dimension of 16 rows per wavefront
+// (a tile of queries), mapped onto v_mfma_f32_16x16x16_bf16:
+//   A fragment: lane l holds A[l%16][4(l/16) .. 4(l/16)+3]
+//   B fragment: lane l holds B[4(l/16) .. 4(l/16)+3][l%16]
+//   D fragment: lane l holds D[4(l/16)+i][l%16], i = 0..3
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kLayerIn[4] = {kGIn, kGHid, kGHid, kGHid};
+constexpr int kLayerOut[4] = {kGHid, kGHid, kGHid, kGOut};
+
+__device__ __forceinline__ v4s ld4(const uint16_t* p) {
+  uint2 v = *reinterpret_cast<const uint2*>(p);
+  v4s r;
+  r[0] = (short)(v.x & 0xffffu);
+  r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xffffu);
+  r[3] = (short)(v.y >> 16);
+  return r;
+}
+__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// The forward weights in LDS, [out][in] per layer with rows padded by 8 bf16
+// so that the 16 rows of a fragment load fall in different banks.
+constexpr int kWPad = 8;
+constexpr int kWOff[4] = {0, kGHid * (kGIn + kWPad), kGHid * (kGIn + kWPad) + kGHid * (kGHid + kWPad),
+                          kGHid * (kGIn + kWPad) + 2 * kGHid * (kGHid + kWPad)};
+constexpr int kWLds = kWOff[3] + kGOut * (kGHid + kWPad);  // bf16 values
+
+// Block-cooperative copy of the forward weights into LDS (wts: kWLds values).
+__device__ __forceinline__ void stage_weights(const GuideNet& N, uint16_t* wts) {
+  for (int L = 0; L < 4; L++) {
+    const int K = kLayerIn[L], NO = kLayerOut[L];
+    for (int e = threadIdx.x; e < NO * K / 4; e += blockDim.x) {  // 4 bf16 per step
+      const int r = (4 * e) / K, col = (4 * e) % K;
+      *reinterpret_cast<uint2*>(&wts[kWOff[L] + r * (K + kWPad) + col]) =
+          *reinterpret_cast<const uint2*>(N.Wt[L] + (size_t)r * K + col);
+    }
+  }
+}
+
 // Launchers (rx_guide.hip).  `rows` points at a device row count; the
 // kernels grid-stride over 16-row tiles so any count up to `cap` works.
+// Encode + forward of `rows` queries (pos[r * stride], dir[r * stride]):
+// raw outputs into Z4 (48 fp32 per row), or with head != NULL the decoded
+// mixtures (kGHead bytes per row).
+hipError_t guide_infer(const GuideNet& N, const float4* pos, const float4* dir, uint32_t stride, const uint32_t* rows,
+                       uint32_t cap, float* Z4, unsigned char* head, int n_cu, hipStream_t s);
 hipError_t guide_forward(const GuideNet& N, const uint16_t* X, const uint32_t* rows, uint32_t cap, uint16_t* A1,
                          uint16_t* A2, uint16_t* A3, float* Z4, int n_cu, hipStream_t s);
 
@@ -143,6 +227,7 @@ struct GuideState {
   DBuf spos, sdir, swdir, spb, swt, srows;
   uint32_t cap = 0, store_cap = 0;
   int step = 0;
+  uint32_t decay_steps = 0;  // Adam steps of the online warm-up (RX_GUIDE_LR_DECAY)
   float lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
   GuideNet net() const {
     GuideNet n;

@@ -21,25 +21,7 @@
 
 namespace rx {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-constexpr int kLayerIn[4] = {kGIn, kGHid, kGHid, kGHid};
-constexpr int kLayerOut[4] = {kGHid, kGHid, kGHid, kGOut};
 constexpr int kDwChunk = 256;  // rows per weight-gradient partial
-
-__device__ __forceinline__ v4s ld4(const uint16_t* p) {
-  uint2 v = *reinterpret_cast<const uint2*>(p);
-  v4s r;
-  r[0] = (short)(v.x & 0xffffu);
-  r[1] = (short)(v.x >> 16);
-  r[2] = (short)(v.y & 0xffffu);
-  r[3] = (short)(v.y >> 16);
-  return r;
-}
-__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
 
 // ------------------------------------------------------------------ forward
 // X [M x 96] bf16 -> A1..A3 [M x 64] bf16 (post-ReLU), Z4 [M x 48] fp32.
@@ -47,11 +29,6 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
 // bf16 so that a fragment load's 16 rows fall in different banks), then its
 // waves run 16-row tiles: the weights are read from HBM/L2 once per block
 // instead of once per tile.
-constexpr int kWPad = 8;
-constexpr int kWOff[4] = {0, kGHid * (kGIn + kWPad), kGHid * (kGIn + kWPad) + kGHid * (kGHid + kWPad),
-                          kGHid * (kGIn + kWPad) + 2 * kGHid * (kGHid + kWPad)};
-constexpr int kWLds = kWOff[3] + kGOut * (kGHid + kWPad);  // bf16 values
-
 __global__ void __launch_bounds__(256) k_mlp_fwd(GuideNet N, const uint16_t* X, const uint32_t* rows, uint32_t cap,
                                                  uint16_t* A1, uint16_t* A2, uint16_t* A3, float* Z4) {
   __shared__ __attribute__((aligned(16))) uint16_t act[2][4][16 * 64];
@@ -60,14 +37,7 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(GuideNet N, const uint16_t* X, 
   const uint32_t M = min(*rows, cap);
   const uint32_t nTiles = (M + 15) / 16;
   if (blockIdx.x * 4 >= nTiles) return;  // no tile for this block (uniform)
-  for (int L = 0; L < 4; L++) {
-    const int K = kLayerIn[L], NO = kLayerOut[L];
-    for (int e = threadIdx.x; e < NO * K / 4; e += blockDim.x) {  // 4 bf16 per step
-      const int r = (4 * e) / K, col = (4 * e) % K;
-      *reinterpret_cast<uint2*>(&wts[kWOff[L] + r * (K + kWPad) + col]) =
-          *reinterpret_cast<const uint2*>(N.Wt[L] + (size_t)r * K + col);
-    }
-  }
+  stage_weights(N, wts);
   __syncthreads();
   uint16_t* const outs[3] = {A1, A2, A3};
   for (uint32_t tb = blockIdx.x * 4; tb < nTiles; tb += gridDim.x * 4) {
@@ -108,6 +78,150 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(GuideNet N, const uint16_t* X, 
       __syncthreads();
     }
   }
+}
+
+// ------------------------------------------------------------------ inference
+// Encode + forward for queries given as (position, direction to the viewer),
+// one 16-row tile per wave at a time, waves independent of each other once
+// the block has staged the forward weights in LDS.  Each layer computes the
+// transposed product H_L^T = W_L H_{L-1}^T (out x rows) with the weights as
+// the A operand; the D fragment of its output tile mt (lane l holds
+// H^T[16 mt + 4(l/16) + i][l%16]) is then exactly the B fragment of the next
+// layer's k-step mt, so the activations never leave the registers, and each
+// lane computes the encoding features of its own B fragments (row l%16,
+// features 16 ks + 4(l/16) .. +3).  Outputs: the raw fp32 outputs (Z4, 48 per
+// row) or, with DECODE, the decoded mixture as kGHead bytes per row (the
+// renderer's form, GuideHead), staged through LDS so that rows are written
+// whole.  Query row r is pos[r * stride], dir[r * stride].
+constexpr int kFwdBlock = 512;  // 8 waves share one copy of the weights
+constexpr int kFwdWaves = kFwdBlock / 64;
+
+// One layer: acc[mt] = W[16 mt .. , :] * in + bias for the layer's MT output tiles.
+template <int KS, int MT>
+__device__ __forceinline__ void fwd_layer(const uint16_t* w, int ldw, const float* bias, const v4s* in, v4f* acc,
+                                          int lr, int lg) {
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) {
+    v4f a = v4f{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) a = mfma(ld4(w + (mt * 16 + lr) * ldw + ks * 16 + 4 * lg), in[ks], a);
+#pragma unroll
+    for (int i = 0; i < 4; i++) a[i] += bias[mt * 16 + 4 * lg + i];
+    acc[mt] = a;
+  }
+}
+__device__ __forceinline__ v4s relu_bf16(v4f a) {
+  v4s r;
+#pragma unroll
+  for (int i = 0; i < 4; i++) r[i] = (short)f2bf(fmaxf(a[i], 0.0f));
+  return r;
+}
+// LDS written by some lanes of a wave, read by others: the wave's LDS
+// operations complete in order once issued.
+__device__ __forceinline__ void wave_lds_sync() {
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <bool DECODE>
+__global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float4* pos, const float4* dir,
+                                                         uint32_t stride, const uint32_t* rows, uint32_t cap,
+                                                         float* Z4, unsigned char* head) {
+  __shared__ __attribute__((aligned(16))) uint16_t wts[kWLds];
+  __shared__ __attribute__((aligned(16))) float zt[DECODE ? kFwdWaves : 1][16 * kGOut];    // a tile's outputs
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
+  const uint32_t M = min(*rows, cap);
+  const uint32_t nTiles = (M + 15) / 16;
+  if (blockIdx.x * kFwdWaves >= nTiles) return;  // block-uniform
+  stage_weights(N, wts);
+  __syncthreads();
+  for (uint32_t tile = blockIdx.x * kFwdWaves + wave; tile < nTiles; tile += gridDim.x * kFwdWaves) {
+    const uint32_t r0 = tile * 16;
+    const uint32_t gr = min(r0 + (uint32_t)lr, M - 1);
+    const float4 p4 = pos[(size_t)gr * stride], d4 = dir[(size_t)gr * stride];
+    const Vec3 x = v3(p4.x, p4.y, p4.z), dd = v3(d4.x, d4.y, d4.z);
+    v4s in[6];
+#pragma unroll
+    for (int ks = 0; ks < 6; ks++) {  // features 16 ks + 4 lg .. +3 = pairs 8 ks + 2 lg, +1
+      const uint32_t p0 = guide_pair(N, x, dd, 8 * ks + 2 * lg), p1 = guide_pair(N, x, dd, 8 * ks + 2 * lg + 1);
+      in[ks] = v4s{(short)(p0 & 0xffffu), (short)(p0 >> 16), (short)(p1 & 0xffffu), (short)(p1 >> 16)};
+    }
+    v4f acc[4];
+    v4s h[4];
+    fwd_layer<6, 4>(wts + kWOff[0], kGIn + kWPad, N.b[0], in, acc, lr, lg);
+#pragma unroll
+    for (int k = 0; k < 4; k++) h[k] = relu_bf16(acc[k]);
+    fwd_layer<4, 4>(wts + kWOff[1], kGHid + kWPad, N.b[1], h, acc, lr, lg);
+#pragma unroll
+    for (int k = 0; k < 4; k++) h[k] = relu_bf16(acc[k]);
+    fwd_layer<4, 4>(wts + kWOff[2], kGHid + kWPad, N.b[2], h, acc, lr, lg);
+#pragma unroll
+    for (int k = 0; k < 4; k++) h[k] = relu_bf16(acc[k]);
+    fwd_layer<4, 3>(wts + kWOff[3], kGHid + kWPad, N.b[3], h, acc, lr, lg);
+    if (!DECODE) {  // row lr, outputs 16 mt + 4 lg .. +3
+      if (r0 + (uint32_t)lr < M)
+#pragma unroll
+        for (int mt = 0; mt < 3; mt++)
+          *reinterpret_cast<float4*>(Z4 + (size_t)(r0 + lr) * kGOut + mt * 16 + 4 * lg) =
+              make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
+      continue;
+    }
+    float* z = zt[DECODE ? wave : 0];
+#pragma unroll
+    for (int mt = 0; mt < 3; mt++)
+      *reinterpret_cast<float4*>(z + lr * kGOut + mt * 16 + 4 * lg) =
+          make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
+    wave_lds_sync();
+    {  // four lanes per row, two lobes each, written to the row's lobe records directly
+      const int r = lane >> 2, sub = lane & 3;
+      const float* zr = z + r * kGOut;
+      const bool out = r0 + (uint32_t)r < M;
+      unsigned char* hrow = head + (size_t)(r0 + r) * kGHead;
+      GuideLobe* lob = reinterpret_cast<GuideLobe*>(hrow);
+      const int j0 = 2 * sub, j1 = 2 * sub + 1;
+      const float l0 = zr[5 * j0], l1 = zr[5 * j1];
+      float mx = fmaxf(l0, l1);
+      mx = fmaxf(mx, __shfl_xor(mx, 1));
+      mx = fmaxf(mx, __shfl_xor(mx, 2));
+      const float e0 = __expf(l0 - mx), e1 = __expf(l1 - mx);
+      float sum = e0 + e1;
+      sum += __shfl_xor(sum, 1);
+      sum += __shfl_xor(sum, 2);
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int j = u ? j1 : j0;
+        const float m0 = zr[5 * j + 1], m1 = zr[5 * j + 2], m2 = zr[5 * j + 3];
+        const float l = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
+        GuideLobe L;
+        L.mu[0] = (half_t)(m0 / l);
+        L.mu[1] = (half_t)(m1 / l);
+        L.mu[2] = (half_t)(m2 / l);
+        L.kappa = (half_t)__expf(fminf(8.0f, fmaxf(-3.0f, zr[5 * j + 4])));
+        const float k = (float)L.kappa;  // the normaliser of the stored concentration
+        L.pi = (u ? e1 : e0) / sum;
+        L.norm = k / (6.28318530717958647692f * (1.0f - __expf(-2.0f * k)));
+        if (out) lob[j] = L;
+      }
+      if (sub == 0 && out)
+        *reinterpret_cast<float4*>(hrow + 8 * sizeof(GuideLobe)) =
+            make_float4(kAlphaMin + kAlphaSpan / (1.0f + __expf(-zr[40])), 0.0f, 0.0f, 0.0f);
+    }
+    wave_lds_sync();  // the tile's outputs are read before the next tile overwrites them
+  }
+}
+
+hipError_t guide_infer(const GuideNet& N, const float4* pos, const float4* dir, uint32_t stride, const uint32_t* rows,
+                       uint32_t cap, float* Z4, unsigned char* head, int n_cu, hipStream_t s) {
+  const uint32_t tiles = (cap + 15) / 16;
+  const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves,
+                                                                   (uint32_t)n_cu * 2));
+  if (head)
+    hipLaunchKernelGGL(k_guide_fwd<true>, dim3(blocks), dim3(kFwdBlock), 0, s, N, pos, dir, stride, rows, cap, Z4,
+                       head);
+  else
+    hipLaunchKernelGGL(k_guide_fwd<false>, dim3(blocks), dim3(kFwdBlock), 0, s, N, pos, dir, stride, rows, cap, Z4,
+                       head);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ backward deltas
@@ -260,14 +374,14 @@ __global__ void __launch_bounds__(256) k_guide_loss(const float* Z4, const float
       q += pi[j] * v[j];
     }
     const float sig = 1.0f / (1.0f + expf(-zz[40]));
-    const float alpha = 0.05f + 0.9f * sig;
+    const float alpha = kAlphaMin + kAlphaSpan * sig;
     const float pbr = pb[r], wr = wt[r];
     const float pm = fmaxf(alpha * q + (1.0f - alpha) * pbr, 1e-12f);
     loss = -wr * logf(pm);
     float g[kGOut];
     for (int c = 0; c < kGOut; c++) g[c] = 0.0f;
     const float gp = -wr / pm, dq = gp * alpha;
-    g[40] = gp * (q - pbr) * 0.9f * sig * (1.0f - sig);
+    g[40] = gp * (q - pbr) * kAlphaSpan * sig * (1.0f - sig);
     for (int j = 0; j < kGLobes; j++) {
       g[5 * j] = dq * pi[j] * (v[j] - q);
       const float dv = dq * pi[j], k = kap[j];
@@ -478,6 +592,8 @@ static int guide_train_device(rx_ctx* c, GuideState& G, uint32_t cap, float* gra
   A.rows = rows;
   A.cap = cap;
   A.lr = G.prm.learning_rate;
+  if ((G.prm.flags & RX_GUIDE_LR_DECAY) && G.decay_steps)  // geometric decay to a tenth over the warm-up
+    A.lr *= powf(0.1f, std::min(1.0f, (float)(G.step - 1) / (float)G.decay_steps));
   A.b1 = G.prm.beta1;
   A.b2 = G.prm.beta2;
   A.eps = G.prm.eps;
@@ -621,9 +737,9 @@ int rx_guide_eval(rx_ctx* c, uint32_t n, const float* pos, const float* dir, flo
   GuideState& G = *c->guide;
   if (int r = guide_reserve(c, G, n)) return r;
   if (int r = upload_queries(c, G, n, pos, dir)) return r;
-  if (int r = hip_err(c, guide_forward(G.net(), G.X.as<uint16_t>(), G.rows.as<uint32_t>(), n, nullptr, nullptr, nullptr,
-                                       G.Z4.as<float>(), c->n_cu, c->stream),
-                      "guide forward"))
+  if (int r = hip_err(c, guide_infer(G.net(), G.pos.as<float4>(), G.dir.as<float4>(), 1, G.rows.as<uint32_t>(), n,
+                                     G.Z4.as<float>(), nullptr, c->n_cu, c->stream),
+                      "guide inference"))
     return r;
   if (int r = hip_err(c, hipMemcpyAsync(out48, G.Z4.p, (size_t)n * kGOut * 4, hipMemcpyDeviceToHost, c->stream), "download"))
     return r;

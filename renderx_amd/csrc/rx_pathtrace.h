@@ -34,18 +34,26 @@ constexpr int kShadowMinWaves = RX_SHADOW_MIN_WAVES;  // the same for the shadow
 constexpr int kMaxBounce = 64;
 constexpr int kShadowBase = 96;       // cnt[kShadowBase + b]: shadow queue length of bounce b
 constexpr int kSpareWords = 192;      // cnt[192, 256): single counters of other passes
-constexpr int kGuideRowBase = 324;   // cnt[kGuideRowBase + b]: network rows of bounce b (guided mode)
+constexpr int kGuideRowBase = 324;    // cnt[kGuideRowBase + b]: network rows of bounce b (guided mode)
 // work counter blocks (kWorkWords each): extend of bounce b at kWorkBase + b * kWorkWords,
 // shadow of bounce b at kShadowWorkBase + b * kWorkWords, one spare block after them
 constexpr int kWorkBase = 512;
 constexpr int kShadowWorkBase = kWorkBase + kMaxBounce * (int)kWorkWords;
 constexpr int kSpareWork = kShadowWorkBase + kMaxBounce * (int)kWorkWords;
 constexpr int kCntWords = kSpareWork + (int)kWorkWords;
-static_assert(kGuideRowBase + kMaxBounce <= kWorkBase, "counter regions do not overlap");
+static_assert(kShadowBase + kMaxBounce <= kSpareWords && kSpareWords + 64 <= kGuideRowBase &&
+                  kGuideRowBase + kMaxBounce <= kWorkBase,
+              "counter regions do not overlap");
 constexpr uint32_t kDefaultPathSlots = 1u << 25;  // rx_render_tuning.path_slots default (~5.7 GB of path state)
-constexpr uint32_t kGuidedPathSlots = 1u << 22;   // guided mode: the network's per-row buffers scale with the slots
 constexpr int kMaxCross = 4;          // index-matched boundary crossings per shadow ray
 constexpr uint32_t FLAG_MIS = 1u;     // last scatter was RANDOM (only transparent crossings since)
+// Guided mode: bits 8.. of the flags count the path's guided vertices so far.
+// Only the first kMaxGuided scattering vertices of a path are guided: a
+// vertex's weight f / p_mix can exceed the plain strategy's by up to
+// 1 / (1 - alpha) where the mixture misses, and over a long random walk in a
+// medium such factors compound into heavy-tailed throughputs.
+constexpr uint32_t kGuidedShift = 8;
+constexpr uint32_t kMaxGuided = 3;
 
 struct DevObject {
   int32_t type, vol;
@@ -164,15 +172,16 @@ __device__ __forceinline__ PathCtx path_ctx(const PassArgs& P, const BatchArgs& 
   return c;
 }
 
-// Guided mode: network outputs per queue row of the current bounce and the
-// training records of the current pass (16 floats per (path, bounce) slot:
-// x, -d, w, p_mix(w), p_bsdf(w), lum L before the vertex, lum throughput
-// arriving at the vertex, RR probability, lum of the vertex's own NEE, tag).
+// Guided mode: the decoded network output of each queue entry of the current
+// bounce (kGHead bytes per network row) and the training
+// records of the current pass (16 floats per (path, bounce) slot: x, -d, w,
+// p_mix(w), p_bsdf(w), lum L before the vertex, lum throughput arriving at
+// the vertex, RR probability, lum of the vertex's own NEE, tag).
 struct GuideArgs {
   GuideNet net;
-  const float* Z4;
-  uint16_t* X;
-  uint32_t* row;  // network row of each queue entry of the bounce (kNoRow: not guided)
+  float4* query;  // per network row: vertex position, direction to the viewer
+  unsigned char* head;  // per network row: decoded output (kGHead bytes)
+  uint32_t* row;  // network row of each queue entry of the bounce (kNoRow: not a guided vertex)
   float* rec;
   uint32_t R;
   float tag;  // > 0: record this pass with that tag
@@ -448,14 +457,17 @@ __global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend(DevTree T, 
 // ------------------------------------------------------------------ guide queries
 // Encode (hit point, view direction) of every queued path of bounce b; row i
 // of the network batch belongs to queue entry i.
-// Whether the shade kernel's scatter at this hit will be a guided surface
-// scatter: the object resolution of its level loop (media, delegation from
-// a refractive inside object) without the scatter itself.
+// Whether the shade kernel's scatter at this hit will be a guided scatter (a
+// diffuse, glossy or Phong surface, or a scattering event inside a medium):
+// the object resolution of its level loop (media, delegation from a
+// refractive inside object) without the scatter itself.  tv receives the
+// vertex's distance along the ray (the hit, or the medium's sampled distance).
 template <bool VOL>
 __device__ __forceinline__ bool guided_vertex(const DevScene& S, int inside, int hitObj, int pos, float t, Vec3 d,
-                                              Vec3 hn, float u5) {
+                                              Vec3 hn, float u5, uint32_t flags, float& tv) {
   int cur = inside != NONE ? inside : hitObj;
-  if (cur == NONE) return false;
+  tv = t;
+  if (cur == NONE || (flags >> kGuidedShift) >= kMaxGuided) return false;
   for (int level = 0; level < 3; level++) {
     const DevObject& O = S.objs[cur];
     if (VOL && O.type == OBJ_VPM) {
@@ -466,7 +478,11 @@ __device__ __forceinline__ bool guided_vertex(const DevScene& S, int inside, int
         cur = hitObj;
         continue;
       }
-      return false;  // boundary crossing, medium scatter or nothing
+      if (be_in && !out_of_vol) {  // a scattering event inside the medium
+        tv = sampDist;
+        return true;
+      }
+      return false;  // boundary crossing or nothing
     }
     if (O.type == OBJ_REFRACTIVE && hitObj != cur && inside == cur) {
       if (pos < 0) return false;
@@ -479,9 +495,8 @@ __device__ __forceinline__ bool guided_vertex(const DevScene& S, int inside, int
   return false;
 }
 
-// Guided mode: the network's inputs for the bounce's guided vertices only,
-// compacted into rows [0, cnt[kGuideRowBase + b]); row[i] maps queue entry i
-// to its row.
+// Guided mode: the network's inputs of path slot p at bounce b, the vertex
+// position and the direction it is seen along; true when the vertex is guided.
 template <bool VOL>
 __device__ __forceinline__ bool guide_input(const DevScene& S, const PathBufs& B, const PassArgs& P, uint32_t b,
                                             uint32_t p, Vec3& x, Vec3& dd) {
@@ -498,10 +513,17 @@ __device__ __forceinline__ bool guide_input(const DevScene& S, const PathBufs& B
     hn = xyz(S.tree.tn[pos]);
   }
   dd = xyz(d);
-  x = xyz(o) + dd * (pos >= 0 ? t : 0.0f);
   const float u5 = VOL ? u01(philox(U4{smp, b, 1u, 0u}, pix, P.key1).y) : 0.0f;
-  return guided_vertex<VOL>(S, B.s[p].x, hitObj, pos, t, dd, hn, u5);
+  float tv;
+  const int4 st = B.s[p];
+  const bool g = guided_vertex<VOL>(S, st.x, hitObj, pos, t, dd, hn, u5, (uint32_t)st.w, tv);
+  x = xyz(o) + dd * tv;
+  return g;
 }
+// The network's queries of bounce b: the guided vertices of the queue get
+// rows [0, cnt[kGuideRowBase + b]) (two float4 each: position, direction to
+// the viewer), appended through an LDS list per block; row[i] maps queue
+// entry i to its row or kNoRow.
 template <bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_guide_query(DevScene S, PathBufs B, PassArgs P, GuideArgs GA, uint32_t b) {
   __shared__ uint32_t list[kStageCap], counts[2];
@@ -525,13 +547,14 @@ __global__ void __launch_bounds__(kRBlock) k_guide_query(DevScene S, PathBufs B,
     __syncthreads();
     staged_claim(qg, &B.cnt[kGuideRowBase + b]);
     __syncthreads();
-    // the staged entries get rows base + k; their inputs are read again (cached)
+    // the staged entries get rows base + k; their vertices are computed again (cached inputs)
     for (uint32_t k = threadIdx.x; k < counts[0]; k += blockDim.x) {
       const uint32_t i = list[k], r = counts[1] + k;
       Vec3 x, dd;
       guide_input<VOL>(S, B, P, b, q[i], x, dd);
       GA.row[i] = r;
-      guide_encode(GA.net, x, -dd, GA.X + (size_t)r * kGIn);
+      GA.query[2 * (size_t)r] = f4(x, 0.0f);
+      GA.query[2 * (size_t)r + 1] = f4(-dd, 0.0f);
     }
     __syncthreads();
   }
@@ -878,18 +901,35 @@ __global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WA
               done = true;
             } else if (be_in && !out_of_vol) {  // scatter inside the medium
               sc.x = o + d * sampDist;
-              Frame lf = frame_from(d);
-              const float g = O.g;
-              float cosT;
-              if (fabsf(g) < 1e-5f) cosT = 1 - 2 * u[6];
-              else {
-                float sq = (1 - g * g) / (1 - g + 2 * g * u[6]);
-                cosT = (1 + g * g - sq * sq) / (2 * g);
+              // guided mode: with probability alpha the learned mixture proposes the direction,
+              // otherwise the phase function does; one-sample MIS over both densities
+              const bool gm = MODE == 2 && GA.row[i] != kNoRow;
+              float ug[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+              if (gm) {
+                guide_head(GA.head + (size_t)GA.row[i] * kGHead, head);
+                U4 r3 = philox(U4{smp, b, 3u, 0u}, pix, pc.key1);
+                ug[0] = u01(r3.x);
+                ug[1] = u01(r3.y);
+                ug[2] = u01(r3.z);
+                ug[3] = u01(r3.w);
               }
-              float sinT = sqrtf(fmaxf(0.0f, 1 - cosT * cosT));
-              float sp, cp;
-              __sincosf(6.28318530717958647692f * u[7], &sp, &cp);
-              Vec3 w = normalize(to_world(lf, v3(sinT * cp, cosT, sinT * sp)));
+              const float g = O.g;
+              Vec3 w;
+              if (gm && ug[0] < head.alpha) {
+                w = guide_sample(head, ug[1], ug[2], ug[3]);
+              } else {
+                Frame lf = frame_from(d);
+                float cosT;
+                if (fabsf(g) < 1e-5f) cosT = 1 - 2 * u[6];
+                else {
+                  float sq = (1 - g * g) / (1 - g + 2 * g * u[6]);
+                  cosT = (1 + g * g - sq * sq) / (2 * g);
+                }
+                float sinT = sqrtf(fmaxf(0.0f, 1 - cosT * cosT));
+                float sp, cp;
+                __sincosf(6.28318530717958647692f * u[7], &sp, &cp);
+                w = normalize(to_world(lf, v3(sinT * cp, cosT, sinT * sp)));
+              }
               float ph = hg(g, dot(d, w));
               sc.inside = cur;
               sc.contact = NONE;
@@ -897,10 +937,19 @@ __global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WA
               sc.neeKind = 2;
               sc.neeObj = cur;
               sc.originProb = p_medium(O, sampDist);
-              if (u[2] < O.albedo || !russian) {
+              float pdir = ph;
+              if (gm) {
+                pdir = head.alpha * guide_pdf(head, w) + (1.0f - head.alpha) * ph;
+                sc.guided = true;
+                sc.pmix = pdir;
+                sc.pb = ph;
+                sc.qrr = russian ? O.albedo : 1.0f;
+              }
+              if ((u[2] < O.albedo || !russian) && pdir > 0.0f) {
                 sc.dir = w;
                 sc.color = ld3(O.c1) * ph;
-                sc.dirProb = russian ? O.albedo * ph : ph;
+                sc.dirProb = russian ? O.albedo * pdir : pdir;
+                sc.pdfB = sc.dirProb;
               } else {
                 sc.dir = v3(0, 0, 0);
                 sc.color = v3(0, 0, 0);
@@ -929,7 +978,7 @@ __global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WA
             } else if (MODE == 2 && dot(d, hn) <= 0 &&
                        (S.objs[cur].type == OBJ_DIFFUSE || S.objs[cur].type == OBJ_GLOSSY ||
                         S.objs[cur].type == OBJ_PHONG) && GA.row[i] != kNoRow) {
-              guide_head(GA.Z4 + (size_t)GA.row[i] * kGOut, head);
+              guide_head(GA.head + (size_t)GA.row[i] * kGHead, head);
               U4 r3 = philox(U4{smp, b, 3u, 0u}, pix, pc.key1);
               const float ug[4] = {u01(r3.x), u01(r3.y), u01(r3.z), u01(r3.w)};
               guided_scatter(S.objs[cur], cur, xh, d, hn, pos, inside, russian, head, ug, u[2], sc);
@@ -983,7 +1032,8 @@ __global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WA
               const DevObject& M = S.objs[sc.neeObj];
               float ph = hg(M.g, dot(d, w));
               f = ld3(M.c1) * ph;
-              pB = (russian ? M.albedo : 1.0f) * ph;
+              pB = (russian ? M.albedo : 1.0f) *
+                   (sc.guided ? head.alpha * guide_pdf(head, w) + (1.0f - head.alpha) * ph : ph);
             }
             Vec3 c = beta * f * (cx / (pL + pB)) * ld3(LO.c1);
             if (maxc(c) > 0) {
@@ -1046,6 +1096,8 @@ __global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WA
             } else if (!sc.transparent) {
               flags = 0;
             }
+            const uint32_t nguided = ((uint32_t)s.w >> kGuidedShift) + (sc.guided ? 1u : 0u);
+            flags = (flags & ((1u << kGuidedShift) - 1u)) | (nguided << kGuidedShift);
             B.o[p] = f4(sc.x, 0.0f);
             B.d[p] = f4(sc.dir, 0.0f);
             B.s[p] = make_int4(sc.inside, sc.contact, sc.contactPos, (int)flags);

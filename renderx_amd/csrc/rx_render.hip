@@ -182,8 +182,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   const uint32_t npix = (uint32_t)(cam->width * cam->height);
   // samples per pass: as many as the path-slot budget holds, so that the later
   // bounces (after Russian roulette) still fill the GPU
-  uint32_t slots = c->tuning.path_slots ? c->tuning.path_slots : kDefaultPathSlots;
-  if (prm->mode == RX_MODE_GUIDED) slots = std::min(slots, kGuidedPathSlots);
+  const uint32_t slots = c->tuning.path_slots ? c->tuning.path_slots : kDefaultPathSlots;
   const uint32_t S_max = std::max<uint32_t>(1u, std::min<uint32_t>(std::max(prm->spp, 1u), slots / npix));
   const uint32_t N = npix * S_max;  // path slots
 #define RXCK(x, w) \
@@ -277,7 +276,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     batch = std::max(256u, G.prm.batch_size);
     steps = std::max(1u, G.prm.steps_per_pass);
     store_cap = (uint32_t)std::min<uint64_t>((uint64_t)batch * steps, 1u << 26);
-    if (int r = guide_reserve(c, G, std::max(N, batch))) return r;
+    G.decay_steps = train_passes * steps;
+    if (int r = guide_reserve(c, G, batch)) return r;  // the training step's rows
     if (prm->sample_offset < train_passes && prm->spp)
       if (int r = guide_reserve_store(c, G, store_cap)) return r;
     const uint32_t Rs = prm->max_depth + 1;
@@ -290,9 +290,11 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, R.rec.as<double>(), (size_t)npix * Rs * kRecFloats / 2);
     }
     GA.net = G.net();
-    GA.Z4 = G.Z4.as<float>();
-    GA.X = G.X.as<uint16_t>();
+    RXCK(R.ghead.ensure(kGHead * (size_t)N), "alloc guide heads");      // 144 B per path slot
+    RXCK(R.gquery.ensure(32 * (size_t)N), "alloc guide queries");       // 32 B per path slot
     RXCK(R.grow.ensure(4 * (size_t)N), "alloc guide rows");
+    GA.head = R.ghead.as<unsigned char>();
+    GA.query = R.gquery.as<float4>();
     GA.row = R.grow.as<uint32_t>();
     GA.rec = R.rec.as<float>();
     GA.R = Rs;
@@ -335,12 +337,12 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       }
       launches++;
       hipEvent_t e1 = timing ? mark() : nullptr;
-      if (mode == RX_MODE_GUIDED) {  // network inference for this bounce's hit points
+      if (mode == RX_MODE_GUIDED) {  // network inference for this bounce's vertices
         if (vol) hipLaunchKernelGGL(k_guide_query<true>, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, GA, b);
         else hipLaunchKernelGGL(k_guide_query<false>, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, GA, b);
-        RXCK(guide_forward(GA.net, GA.X, B.cnt + kGuideRowBase + b, N, nullptr, nullptr, nullptr,
-                           c->guide->Z4.as<float>(), c->n_cu, s),
-             "guide forward");
+        RXCK(guide_infer(GA.net, GA.query, GA.query + 1, 2, B.cnt + kGuideRowBase + b, P.n_paths, nullptr, GA.head,
+                         c->n_cu, s),
+             "guide inference");
         if (vol) hipLaunchKernelGGL((k_shade<2, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
         else hipLaunchKernelGGL((k_shade<2, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
       } else if (mode == RX_MODE_NEE) {

@@ -26,7 +26,7 @@ struct RenderState {
   SceneBufs scene;
   DBuf o, d, beta, lastp, L, s, hit, q0, q1, sq, cnt, so, sd, sc, ss, sum, sumsq, mean, stats;
   DBuf rec, bcount, total;                // guided-mode training records
-  DBuf grow;                              // guided mode: network row of each queue entry
+  DBuf gquery, ghead, grow;               // guided mode: network query and decoded output per row, row per entry
   DBuf bverts, bnv, bvis, bpairs, bsplat, bown;  // bidirectional tracer: subpath vertices, counts, visibility, splats, own sums
   uint32_t cap = 0;
   float box_lo[3], box_hi[3];
