@@ -313,7 +313,9 @@ int rx_scene_file_get(const rx_scene_file* file, float* verts, uint32_t* tris, r
 
 /* ------------------------------------------------------------------ path guiding */
 typedef struct rx_guide_params {
-  uint32_t train_passes;   /* leading samples per pixel that record training data (one pass each) */
+  uint32_t train_passes;   /* online warm-up passes: each renders enough samples per pixel for
+                              batch_size * steps_per_pass paths (at least one), the leading samples
+                              of every pixel, and trains on them */
   uint32_t batch_size;     /* radiance samples per Adam step (>= 256) */
   uint32_t steps_per_pass; /* Adam steps after each training pass, over deterministic
                               permutations of that pass's samples (a store of at most

@@ -15,7 +15,10 @@ constexpr int kGIn = 96, kGHid = 64, kGOut = 48, kGLobes = 8;
 // guided vertex's samples on the BSDF / phase function bounds the vertex's
 // weight f / p_mix by twice the plain strategy's, so throughputs cannot grow
 // geometrically along a path whose directions the mixture misses.
-constexpr float kAlphaMin = 0.05f, kAlphaSpan = 0.45f;
+#ifndef RX_GUIDE_ALPHA_MAX
+#define RX_GUIDE_ALPHA_MAX 0.5f
+#endif
+constexpr float kAlphaMin = 0.05f, kAlphaSpan = RX_GUIDE_ALPHA_MAX - kAlphaMin;
 constexpr int kGParams = kGIn * kGHid + 2 * kGHid * kGHid + kGHid * kGOut + 3 * kGHid + kGOut;  // 17,648
 
 struct GuideNet {
@@ -25,70 +28,51 @@ struct GuideNet {
   float lo[3], hi[3];     // encoding bounds (scene box)
 };
 
-// Decoded network output of one renderer query, kGHead bytes: per lobe j a
-// 16-byte record (unit mean mu_j as 3 fp16, concentration kappa_j as fp16,
-// mixture weight pi_j as fp32, vMF normaliser kappa_j / (2 pi (1 - e^{-2
-// kappa_j})) of the stored kappa_j as fp32), then the selection probability
-// alpha (fp32).  The sampler and the density read the same stored values:
-// weights divided by their stored sum, means renormalised, so the density is
-// exactly the sampler's.  Approximate reciprocals and square roots are used
-// on purpose: both sides use the same ones.
+// Decoded network output of one renderer query, kGHead bytes, all FP32:
+//   float4 lobe[8]  unit mean mu_j and concentration kappa_j
+//   float  cw[8]    w_j kappa_j / (2 pi (1 - e^{-2 kappa_j})): weight times vMF normaliser
+//   float  cdf[8]   cumulative weights, cdf[7] = 1; w_j = cdf[j] - cdf[j-1]
+//   float  alpha    selection probability of the mixture (then 3 floats of padding)
+// The decode computes the weights from the stored cumulative table, so the
+// sampler (lobe j where u0 < cdf[j] first holds) and the density read the
+// same numbers: the density is exactly the sampler's up to FP32 rounding.
 typedef _Float16 half_t;
-struct __attribute__((aligned(16))) GuideLobe {
-  half_t mu[3], kappa;
-  float pi, norm;
-};
-constexpr int kGHead = 8 * (int)sizeof(GuideLobe) + 16;  // 144 bytes per row
-static_assert(sizeof(GuideLobe) == 16, "lobe record is 16 bytes");
+constexpr int kGHead = 8 * 16 + 8 * 4 + 8 * 4 + 16;  // 208 bytes per row
+constexpr int kGHeadCw = 128, kGHeadCdf = 160, kGHeadAlpha = 192;
 
 struct GuideHead {
-  const GuideLobe* l;
-  float inv, alpha;
+  const float4* lobe;
+  const float* cw;
+  const float* cdf;
+  float alpha;
 };
 
 __device__ __forceinline__ void guide_head(const unsigned char* row, GuideHead& g) {
-  g.l = reinterpret_cast<const GuideLobe*>(row);
-  float s = 0.0f;
-  for (int j = 0; j < kGLobes; j++) s += g.l[j].pi;
-  g.inv = __builtin_amdgcn_rcpf(s);
-  g.alpha = *reinterpret_cast<const float*>(row + 8 * sizeof(GuideLobe));
-}
-__device__ __forceinline__ float guide_weight(const GuideHead& g, int j) { return g.l[j].pi * g.inv; }
-// Lobe j: unit mean, concentration and normaliser.
-__device__ __forceinline__ void guide_lobe(const GuideHead& g, int j, Vec3& mu, float& kappa, float& norm) {
-  const GuideLobe L = g.l[j];
-  const float m0 = (float)L.mu[0], m1 = (float)L.mu[1], m2 = (float)L.mu[2];
-  const float r = __builtin_amdgcn_rsqf(fmaxf(m0 * m0 + m1 * m1 + m2 * m2, 1e-12f));
-  mu = v3(m0 * r, m1 * r, m2 * r);
-  kappa = (float)L.kappa;
-  norm = L.norm;
+  g.lobe = reinterpret_cast<const float4*>(row);
+  g.cw = reinterpret_cast<const float*>(row + kGHeadCw);
+  g.cdf = reinterpret_cast<const float*>(row + kGHeadCdf);
+  g.alpha = *reinterpret_cast<const float*>(row + kGHeadAlpha);
 }
 
 __device__ __forceinline__ float guide_pdf(const GuideHead& h, Vec3 w) {
   float q = 0.0f;
 #pragma unroll
   for (int j = 0; j < kGLobes; j++) {
-    Vec3 mu;
-    float k, norm;
-    guide_lobe(h, j, mu, k, norm);
-    const float c = mu.x * w.x + mu.y * w.y + mu.z * w.z;
-    q += h.l[j].pi * norm * __expf(k * (c - 1.0f));
+    const float4 L = h.lobe[j];
+    const float c = L.x * w.x + L.y * w.y + L.z * w.z;
+    q += h.cw[j] * __expf(L.w * (c - 1.0f));
   }
-  return q * h.inv;
+  return q;
 }
 
 // Lobe pick by u0, then the numerically stable vMF sampler (W. Jakob,
 // "Numerically stable sampling of the von Mises Fisher distribution on S^2", 2012).
 __device__ __forceinline__ Vec3 guide_sample(const GuideHead& h, float u0, float u1, float u2) {
   int j = 0;
-  float c = guide_weight(h, 0);
-  while (j < kGLobes - 1 && u0 >= c) {
-    j++;
-    c += guide_weight(h, j);
-  }
-  Vec3 m;
-  float k, norm;
-  guide_lobe(h, j, m, k, norm);
+  while (j < kGLobes - 1 && u0 >= h.cdf[j]) j++;
+  const float4 L = h.lobe[j];
+  const Vec3 m = v3(L.x, L.y, L.z);
+  const float k = L.w;
   float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * k)) * __builtin_amdgcn_rcpf(k);
   w = fminf(1.0f, fmaxf(-1.0f, w));
   float r = sqrtf(fmaxf(0.0f, 1.0f - w * w));
@@ -208,8 +192,7 @@ __device__ __forceinline__ void stage_weights(const GuideNet& N, uint16_t* wts) 
 // mixtures (kGHead bytes per row).
 hipError_t guide_infer(const GuideNet& N, const float4* pos, const float4* dir, uint32_t stride, const uint32_t* rows,
                        uint32_t cap, float* Z4, unsigned char* head, int n_cu, hipStream_t s);
-hipError_t guide_forward(const GuideNet& N, const uint16_t* X, const uint32_t* rows, uint32_t cap, uint16_t* A1,
-                         uint16_t* A2, uint16_t* A3, float* Z4, int n_cu, hipStream_t s);
+
 
 }  // namespace rx
 
@@ -222,10 +205,10 @@ namespace rx {
 struct GuideState {
   rx_guide_params prm{};
   DBuf params, adam_m, adam_v, grad, Wt[4], Wr[4];
-  DBuf X, A1, A2, A3, Z4, dZ4, D1, D2, D3, partial[4], loss_part, rows, pos, dir, wdir, pb, wt;
+  DBuf Z4, tpart, loss_part, rows, pos, dir, wdir, pb, wt;  // rx_guide_eval outputs, training partials, step rows
   // one training pass's radiance samples, mini-batches are gathered from here
   DBuf spos, sdir, swdir, spb, swt, srows;
-  uint32_t cap = 0, store_cap = 0;
+  uint32_t cap = 0, store_cap = 0, train_blocks = 1;
   int step = 0;
   uint32_t decay_steps = 0;  // Adam steps of the online warm-up (RX_GUIDE_LR_DECAY)
   float lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};

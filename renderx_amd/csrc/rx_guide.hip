@@ -1,13 +1,9 @@
-// Path-guiding network on gfx950 matrix cores: BF16 MFMA forward, backward
-// and weight-gradient kernels with FP32 accumulation, deterministic gradient
-// reduction and an FP32 Adam step.  No counterpart in the reference.
-//
-// Every product in the network has one dimension of 16 rows per wavefront
-// (a tile of queries), so all of it maps onto v_mfma_f32_16x16x16_bf16:
-//   A fragment: lane l holds A[l%16][4(l/16) .. 4(l/16)+3]
-//   B fragment: lane l holds B[4(l/16) .. 4(l/16)+3][l%16]
-//   D fragment: lane l holds D[4(l/16)+i][l%16], i = 0..3
-// Activations move between layers through a per-wave LDS tile.
+// Path-guiding network on gfx950 matrix cores: the renderer's inference
+// (encode + forward + decode) and the fused training step (forward, loss,
+// backward and weight gradients in one kernel, then a deterministic
+// block-order reduction with the FP32 Adam update), all products on
+// v_mfma_f32_16x16x16_bf16 with FP32 accumulation (fragment layout in
+// rx_guide.h).  No counterpart in the reference.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,64 +17,6 @@
 
 namespace rx {
 
-constexpr int kDwChunk = 256;  // rows per weight-gradient partial
-
-// ------------------------------------------------------------------ forward
-// X [M x 96] bf16 -> A1..A3 [M x 64] bf16 (post-ReLU), Z4 [M x 48] fp32.
-// Each block stages the four weight matrices in LDS once (rows padded by 8
-// bf16 so that a fragment load's 16 rows fall in different banks), then its
-// waves run 16-row tiles: the weights are read from HBM/L2 once per block
-// instead of once per tile.
-__global__ void __launch_bounds__(256) k_mlp_fwd(GuideNet N, const uint16_t* X, const uint32_t* rows, uint32_t cap,
-                                                 uint16_t* A1, uint16_t* A2, uint16_t* A3, float* Z4) {
-  __shared__ __attribute__((aligned(16))) uint16_t act[2][4][16 * 64];
-  __shared__ __attribute__((aligned(16))) uint16_t wts[kWLds];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
-  const uint32_t M = min(*rows, cap);
-  const uint32_t nTiles = (M + 15) / 16;
-  if (blockIdx.x * 4 >= nTiles) return;  // no tile for this block (uniform)
-  stage_weights(N, wts);
-  __syncthreads();
-  uint16_t* const outs[3] = {A1, A2, A3};
-  for (uint32_t tb = blockIdx.x * 4; tb < nTiles; tb += gridDim.x * 4) {
-    const uint32_t tile = tb + wave;
-    const uint32_t row0 = tile * 16;
-    const uint32_t rowA = min(row0 + (uint32_t)lr, M ? M - 1 : 0u);
-    const bool live = tile < nTiles;
-    for (int L = 0; L < 4; L++) {
-      const int K = kLayerIn[L], NO = kLayerOut[L], NT = NO / 16;
-      v4f acc[4];
-      for (int nt = 0; nt < 4; nt++) acc[nt] = v4f{0, 0, 0, 0};
-      const uint16_t* ain = (L == 0) ? nullptr : act[(L - 1) & 1][wave];
-      const uint16_t* wl = wts + kWOff[L];
-      for (int ks = 0; ks < K / 16; ks++) {
-        v4s a = live ? ((L == 0) ? ld4(X + (size_t)rowA * kGIn + ks * 16 + 4 * lg) : ld4(ain + lr * 64 + ks * 16 + 4 * lg))
-                     : v4s{0, 0, 0, 0};
-        for (int nt = 0; nt < NT; nt++) {
-          v4s b = ld4(wl + (nt * 16 + lr) * (K + kWPad) + ks * 16 + 4 * lg);
-          acc[nt] = mfma(a, b, acc[nt]);
-        }
-      }
-      for (int nt = 0; nt < NT; nt++) {
-        const int col = nt * 16 + lr;
-        const float bias = N.b[L][col];
-        for (int i = 0; i < 4; i++) {
-          const int row = 4 * lg + i;
-          const float z = acc[nt][i] + bias;
-          const uint32_t gr = row0 + row;
-          if (L < 3) {
-            const uint16_t h = f2bf(fmaxf(z, 0.0f));
-            act[L & 1][wave][row * 64 + col] = h;
-            if (live && gr < M && outs[L]) outs[L][(size_t)gr * kGHid + col] = h;
-          } else if (live && gr < M) {
-            Z4[(size_t)gr * kGOut + col] = z;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
 
 // ------------------------------------------------------------------ inference
 // Encode + forward for queries given as (position, direction to the viewer),
@@ -128,7 +66,7 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
                                                          uint32_t stride, const uint32_t* rows, uint32_t cap,
                                                          float* Z4, unsigned char* head) {
   __shared__ __attribute__((aligned(16))) uint16_t wts[kWLds];
-  __shared__ __attribute__((aligned(16))) float zt[DECODE ? kFwdWaves : 1][16 * kGOut];    // a tile's outputs
+  __shared__ __attribute__((aligned(16))) float zt[DECODE ? kFwdWaves : 1][16 * kGOut];  // a tile's outputs
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
   const uint32_t M = min(*rows, cap);
   const uint32_t nTiles = (M + 15) / 16;
@@ -172,38 +110,43 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
       *reinterpret_cast<float4*>(z + lr * kGOut + mt * 16 + 4 * lg) =
           make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
     wave_lds_sync();
-    {  // four lanes per row, two lobes each, written to the row's lobe records directly
+    {  // four lanes per row, two lobes each, written to the row's decoded record directly
       const int r = lane >> 2, sub = lane & 3;
       const float* zr = z + r * kGOut;
       const bool out = r0 + (uint32_t)r < M;
       unsigned char* hrow = head + (size_t)(r0 + r) * kGHead;
-      GuideLobe* lob = reinterpret_cast<GuideLobe*>(hrow);
       const int j0 = 2 * sub, j1 = 2 * sub + 1;
       const float l0 = zr[5 * j0], l1 = zr[5 * j1];
       float mx = fmaxf(l0, l1);
       mx = fmaxf(mx, __shfl_xor(mx, 1));
       mx = fmaxf(mx, __shfl_xor(mx, 2));
       const float e0 = __expf(l0 - mx), e1 = __expf(l1 - mx);
-      float sum = e0 + e1;
-      sum += __shfl_xor(sum, 1);
-      sum += __shfl_xor(sum, 2);
+      // cumulative sums over the row's lanes in lobe order (exclusive prefix of the pair sums)
+      const float pair = e0 + e1;
+      const float p1 = __shfl_xor(pair, 1), p2 = __shfl_xor(pair, 2), p3 = __shfl_xor(pair, 3);
+      const float total = pair + p1 + p2 + p3;
+      // lanes sub = 0..3 hold pairs 0..3; the sum of the pairs before this lane's
+      const float before = (sub == 0) ? 0.0f : (sub == 1) ? p1 : (sub == 2) ? (p2 + p3) : (p1 + p2 + p3);
+      const float inv = 1.0f / total;
+      const float cdf0 = (before + e0) * inv;
+      const float cdf1 = sub == 3 ? 1.0f : (before + e0 + e1) * inv;
+      const float cprev = before * inv;
+      const float wts2[2] = {cdf0 - cprev, cdf1 - cdf0};
+      const float cdfs[2] = {cdf0, cdf1};
 #pragma unroll
       for (int u = 0; u < 2; u++) {
         const int j = u ? j1 : j0;
         const float m0 = zr[5 * j + 1], m1 = zr[5 * j + 2], m2 = zr[5 * j + 3];
         const float l = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
-        GuideLobe L;
-        L.mu[0] = (half_t)(m0 / l);
-        L.mu[1] = (half_t)(m1 / l);
-        L.mu[2] = (half_t)(m2 / l);
-        L.kappa = (half_t)__expf(fminf(8.0f, fmaxf(-3.0f, zr[5 * j + 4])));
-        const float k = (float)L.kappa;  // the normaliser of the stored concentration
-        L.pi = (u ? e1 : e0) / sum;
-        L.norm = k / (6.28318530717958647692f * (1.0f - __expf(-2.0f * k)));
-        if (out) lob[j] = L;
+        const float k = __expf(fminf(8.0f, fmaxf(-3.0f, zr[5 * j + 4])));
+        if (out) {
+          reinterpret_cast<float4*>(hrow)[j] = make_float4(m0 / l, m1 / l, m2 / l, k);
+          reinterpret_cast<float*>(hrow + kGHeadCw)[j] = wts2[u] * (k / (6.28318530717958647692f * (1.0f - __expf(-2.0f * k))));
+          reinterpret_cast<float*>(hrow + kGHeadCdf)[j] = cdfs[u];
+        }
       }
       if (sub == 0 && out)
-        *reinterpret_cast<float4*>(hrow + 8 * sizeof(GuideLobe)) =
+        *reinterpret_cast<float4*>(hrow + kGHeadAlpha) =
             make_float4(kAlphaMin + kAlphaSpan / (1.0f + __expf(-zr[40])), 0.0f, 0.0f, 0.0f);
     }
     wave_lds_sync();  // the tile's outputs are read before the next tile overwrites them
@@ -224,192 +167,12 @@ hipError_t guide_infer(const GuideNet& N, const float4* pos, const float4* dir, 
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ backward deltas
-// dZ4 [M x 48] bf16 -> D3, D2, D1 [M x 64] bf16:  D_l = (D_{l+1} W_{l+1}^T) * (A_l > 0)
-__global__ void __launch_bounds__(256) k_mlp_bwd(GuideNet N, const uint16_t* dZ4, const uint32_t* rows, uint32_t cap,
-                                                 const uint16_t* A1, const uint16_t* A2, const uint16_t* A3,
-                                                 uint16_t* D1, uint16_t* D2, uint16_t* D3) {
-  __shared__ __attribute__((aligned(16))) uint16_t dl[2][4][16 * 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
-  const uint32_t M = min(*rows, cap);
-  const uint32_t nTiles = (M + 15) / 16;
-  const uint16_t* acts[3] = {A1, A2, A3};
-  uint16_t* outs[3] = {D1, D2, D3};
-  for (uint32_t tb = blockIdx.x * 4; tb < nTiles; tb += gridDim.x * 4) {
-    const uint32_t tile = tb + wave;
-    const uint32_t row0 = tile * 16;
-    const uint32_t rowA = min(row0 + (uint32_t)lr, M ? M - 1 : 0u);
-    const bool live = tile < nTiles;
-    // step s: from layer L+1's delta (K = out of layer L+1) to layer L's delta
-    for (int s = 0; s < 3; s++) {
-      const int L = 2 - s;             // producing D_{L+1} in 1-based terms: D3, D2, D1
-      const int K = kLayerOut[L + 1];  // 48 for the first step, then 64
-      v4f acc[4];
-      for (int nt = 0; nt < 4; nt++) acc[nt] = v4f{0, 0, 0, 0};
-      const uint16_t* din = dl[(s + 1) & 1][wave];
-      for (int ks = 0; ks < K / 16; ks++) {
-        v4s a = live ? ((s == 0) ? ld4(dZ4 + (size_t)rowA * kGOut + ks * 16 + 4 * lg) : ld4(din + lr * 64 + ks * 16 + 4 * lg))
-                     : v4s{0, 0, 0, 0};
-        for (int nt = 0; nt < 4; nt++) {
-          v4s b = ld4(N.Wr[L + 1] + (size_t)(nt * 16 + lr) * K + ks * 16 + 4 * lg);
-          acc[nt] = mfma(a, b, acc[nt]);
-        }
-      }
-      for (int nt = 0; nt < 4; nt++) {
-        const int col = nt * 16 + lr;
-        for (int i = 0; i < 4; i++) {
-          const int row = 4 * lg + i;
-          const uint32_t gr = row0 + row;
-          float v = 0.0f;
-          if (live && gr < M) {
-            const float a = bf2f(acts[L][(size_t)gr * kGHid + col]);
-            v = a > 0.0f ? acc[nt][i] : 0.0f;
-          }
-          const uint16_t h = f2bf(v);
-          dl[s & 1][wave][row * 64 + col] = h;
-          if (live && gr < M) outs[L][(size_t)gr * kGHid + col] = h;
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// ------------------------------------------------------------------ weight gradients
-// partial[c][m*OUT + n] = sum over chunk c's rows of Ain[r][m] * D[r][n];
-// partial[c][IN*OUT + n] = sum of D[r][n].  One wave per chunk of kDwChunk rows.
-template <int IN, int OUT>
-__global__ void __launch_bounds__(256) k_mlp_dw(const uint16_t* Ain, const uint16_t* D, const uint32_t* rows,
-                                                uint32_t cap, float* partial) {
-  constexpr int MT = IN / 16, NT = OUT / 16, P = IN * OUT + OUT;
-  __shared__ __attribute__((aligned(16))) uint16_t ta[4][16 * IN];
-  __shared__ __attribute__((aligned(16))) uint16_t td[4][16 * OUT];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
-  const uint32_t M = min(*rows, cap);
-  const uint32_t nChunks = (M + kDwChunk - 1) / kDwChunk;
-  for (uint32_t cb = blockIdx.x * 4; cb < nChunks; cb += gridDim.x * 4) {
-    const uint32_t chunk = cb + wave;
-    v4f acc[MT][NT];
-    for (int m = 0; m < MT; m++)
-      for (int n = 0; n < NT; n++) acc[m][n] = v4f{0, 0, 0, 0};
-    float dbs = 0.0f;
-    for (int kt = 0; kt < kDwChunk / 16; kt++) {
-      const uint32_t r0 = chunk * kDwChunk + kt * 16;
-      // stage 16 rows of Ain and D (zero beyond M)
-      for (int e = lane; e < 16 * IN / 4; e += 64) {
-        const int r = (e * 4) / IN, c = (e * 4) % IN;
-        uint2 v = make_uint2(0, 0);
-        if (chunk < nChunks && r0 + r < M) v = *reinterpret_cast<const uint2*>(Ain + (size_t)(r0 + r) * IN + c);
-        *reinterpret_cast<uint2*>(&ta[wave][r * IN + c]) = v;
-      }
-      for (int e = lane; e < 16 * OUT / 4; e += 64) {
-        const int r = (e * 4) / OUT, c = (e * 4) % OUT;
-        uint2 v = make_uint2(0, 0);
-        if (chunk < nChunks && r0 + r < M) v = *reinterpret_cast<const uint2*>(D + (size_t)(r0 + r) * OUT + c);
-        *reinterpret_cast<uint2*>(&td[wave][r * OUT + c]) = v;
-      }
-      __syncthreads();
-      v4s b[NT];
-      for (int n = 0; n < NT; n++)
-        for (int s = 0; s < 4; s++) b[n][s] = (short)td[wave][(4 * lg + s) * OUT + n * 16 + lr];
-      for (int m = 0; m < MT; m++) {
-        v4s a;
-        for (int s = 0; s < 4; s++) a[s] = (short)ta[wave][(4 * lg + s) * IN + m * 16 + lr];
-        for (int n = 0; n < NT; n++) acc[m][n] = mfma(a, b[n], acc[m][n]);
-      }
-      if (lane < OUT)
-        for (int r = 0; r < 16; r++) dbs += bf2f(td[wave][r * OUT + lane]);
-      __syncthreads();
-    }
-    if (chunk < nChunks) {
-      float* out = partial + (size_t)chunk * P;
-      for (int m = 0; m < MT; m++)
-        for (int n = 0; n < NT; n++)
-          for (int i = 0; i < 4; i++) out[(m * 16 + 4 * lg + i) * OUT + n * 16 + lr] = acc[m][n][i];
-      if (lane < OUT) out[IN * OUT + lane] = dbs;
-    }
-  }
-}
-
-// ------------------------------------------------------------------ loss
-// Per row: decode the head, the one-sample-MIS mixture density
-// p = alpha q(w) + (1 - alpha) p_bsdf(w), loss -wt log p and its gradient with
-// respect to the 48 outputs (bf16).  loss_part[block] = block sum (fixed order).
-__global__ void __launch_bounds__(256) k_guide_loss(const float* Z4, const float4* wdir, const float* pb,
-                                                    const float* wt, const uint32_t* rows, uint32_t cap,
-                                                    uint16_t* dZ4, float* loss_part) {
-  __shared__ float red[256];
-  const uint32_t M = min(*rows, cap);
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  float loss = 0.0f;
-  if (r < M) {
-    const float* z = Z4 + (size_t)r * kGOut;
-    float zz[kGOut];
-    for (int c = 0; c < kGOut; c++) zz[c] = z[c];
-    float mx = zz[0];
-    for (int j = 1; j < kGLobes; j++) mx = fmaxf(mx, zz[5 * j]);
-    float pi[kGLobes], s = 0.0f;
-    for (int j = 0; j < kGLobes; j++) {
-      pi[j] = expf(zz[5 * j] - mx);
-      s += pi[j];
-    }
-    const float4 w4 = wdir[r];
-    const float om[3] = {w4.x, w4.y, w4.z};
-    float v[kGLobes], mu[kGLobes][3], ml[kGLobes], kap[kGLobes], cj[kGLobes];
-    bool clamped[kGLobes];
-    float q = 0.0f;
-    for (int j = 0; j < kGLobes; j++) {
-      pi[j] /= s;
-      float m0 = zz[5 * j + 1], m1 = zz[5 * j + 2], m2 = zz[5 * j + 3];
-      float l = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
-      ml[j] = l;
-      mu[j][0] = m0 / l;
-      mu[j][1] = m1 / l;
-      mu[j][2] = m2 / l;
-      float lk = zz[5 * j + 4];
-      clamped[j] = lk < -3.0f || lk > 8.0f;
-      kap[j] = expf(fminf(8.0f, fmaxf(-3.0f, lk)));
-      cj[j] = mu[j][0] * om[0] + mu[j][1] * om[1] + mu[j][2] * om[2];
-      v[j] = kap[j] / (6.28318530717958647692f * (1.0f - expf(-2.0f * kap[j]))) * expf(kap[j] * (cj[j] - 1.0f));
-      q += pi[j] * v[j];
-    }
-    const float sig = 1.0f / (1.0f + expf(-zz[40]));
-    const float alpha = kAlphaMin + kAlphaSpan * sig;
-    const float pbr = pb[r], wr = wt[r];
-    const float pm = fmaxf(alpha * q + (1.0f - alpha) * pbr, 1e-12f);
-    loss = -wr * logf(pm);
-    float g[kGOut];
-    for (int c = 0; c < kGOut; c++) g[c] = 0.0f;
-    const float gp = -wr / pm, dq = gp * alpha;
-    g[40] = gp * (q - pbr) * kAlphaSpan * sig * (1.0f - sig);
-    for (int j = 0; j < kGLobes; j++) {
-      g[5 * j] = dq * pi[j] * (v[j] - q);
-      const float dv = dq * pi[j], k = kap[j];
-      const float e = expf(-2.0f * k);
-      const float dlk = 1.0f / k - 2.0f * e / (1.0f - e) + (cj[j] - 1.0f);
-      g[5 * j + 4] = clamped[j] ? 0.0f : dv * v[j] * dlk * k;
-      const float sc = dv * v[j] * k / ml[j];
-      for (int a = 0; a < 3; a++) g[5 * j + 1 + a] = sc * (om[a] - cj[j] * mu[j][a]);
-    }
-    for (int c = 0; c < kGOut; c++) dZ4[(size_t)r * kGOut + c] = f2bf(g[c]);
-  }
-  red[threadIdx.x] = loss;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
-}
-
 // ------------------------------------------------------------------ Adam
 struct AdamArgs {
   float* p;  // fp32 master parameters (W1,W2,W3,W4 row-major in x out, then b1..b4)
   float* m;
   float* v;
   float* grad_out;  // optional: mean gradient
-  const float* partial[4];
-  uint32_t nchunks;
   const uint32_t* rows;
   uint32_t cap;
   float lr, b1, b2, eps, c1, c2;
@@ -440,18 +203,272 @@ __device__ __forceinline__ void param_loc(int idx, int& L, int& off, bool& isb) 
   L = -1;
 }
 
-__global__ void k_adam(AdamArgs A) {
+// ------------------------------------------------------------------ fused training step
+// One Adam step's gradients in one launch, one block of four waves per CU.
+// Each wave takes 16 rows at a time: the encodings are built in registers,
+// the forward and backward chains run in the transposed form of the
+// inference kernel (D fragments of one layer are the B fragments of the
+// next, activations and deltas stay in registers), and the loss and its
+// gradient with respect to the 48 outputs are computed by four lanes per
+// row.  The wave then stores its tile's inputs X^T, activations H_l^T and
+// deltas D_l^T (bf16, feature-major) in LDS; after a block barrier the four
+// waves split the 68 16x16 tiles of the weight gradients, each contracting
+// over the block's 64 rows, and 240 threads sum the bias gradients.  Every
+// block writes its own partial (kGParams floats) and loss sum; k_adam_reduce
+// adds the partials in block order, so the step is deterministic.
+constexpr int kTrainWaves = 4;
+constexpr int kTrainBlock = 64 * kTrainWaves;
+// stored tile rows (features) of one wave: X 96, H1..H3, D1..D3 64 each, dZ 48
+constexpr int kFX = 0, kFH1 = 96, kFH2 = 160, kFH3 = 224, kFD1 = 288, kFD2 = 352, kFD3 = 416, kFDZ = 480,
+              kTileFeat = 528;
+// backward weights in LDS: W_l [in][out] for l = 2, 3, 4 (rows padded by 8)
+constexpr int kRPad = 8;
+constexpr int kROff[3] = {0, kGHid * (kGHid + kRPad), 2 * kGHid * (kGHid + kRPad)};
+constexpr int kRLds = kROff[2] + kGHid * (kGOut + kRPad);
+// weight-gradient tiles: layer 1 4 x 6, layers 2, 3 4 x 4, layer 4 3 x 4 (out x in)
+constexpr int kDwTiles = 24 + 16 + 16 + 12, kDwPerWave = kDwTiles / kTrainWaves;
+static_assert(kDwTiles % kTrainWaves == 0, "weight-gradient tiles split evenly over the waves");
+constexpr int kParamBase[4] = {0, kGIn * kGHid, kGIn * kGHid + kGHid * kGHid, kGIn * kGHid + 2 * kGHid * kGHid};
+constexpr int kBiasBase = kGIn * kGHid + 2 * kGHid * kGHid + kGHid * kGOut;
+
+struct TrainRows {
+  const float4 *pos, *dir, *wdir;
+  const float *pb, *wt;
+};
+
+__device__ __forceinline__ void dw_tile(int t, int& L, int& mo, int& mi) {
+  if (t < 24) { L = 0; mo = t / 6; mi = t % 6; }
+  else if (t < 40) { L = 1; mo = (t - 24) / 4; mi = (t - 24) % 4; }
+  else if (t < 56) { L = 2; mo = (t - 40) / 4; mi = (t - 40) % 4; }
+  else { L = 3; mo = (t - 56) / 4; mi = (t - 56) % 4; }
+}
+
+// A 16 x 16 D fragment (features 16 m + 4 lg + i, row lr) into a feature-major tile.
+__device__ __forceinline__ void store_frag(uint16_t* tile, int f0, v4s v, int lr, int lg) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) tile[(f0 + 4 * lg + i) * 16 + lr] = (uint16_t)v[i];
+}
+
+__global__ void __launch_bounds__(kTrainBlock) k_guide_train(GuideNet N, TrainRows T, const uint32_t* rows,
+                                                             uint32_t cap, float* partial, float* loss_part) {
+  __shared__ __attribute__((aligned(16))) uint16_t wts[kWLds];
+  __shared__ __attribute__((aligned(16))) uint16_t wtr[kRLds];
+  __shared__ __attribute__((aligned(16))) uint16_t tiles[kTrainWaves][kTileFeat * 16];
+  __shared__ __attribute__((aligned(16))) float zt[kTrainWaves][16 * kGOut];
+  __shared__ float lsum[kTrainWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
+  const uint32_t M = min(*rows, cap);
+  const uint32_t nIter = (M + 16 * kTrainWaves - 1) / (16 * kTrainWaves);
+  stage_weights(N, wts);
+  for (int L = 1; L < 4; L++) {  // backward weights, [in][out]
+    const int K = kLayerOut[L], ld = K + kRPad;
+    for (int e = threadIdx.x; e < kGHid * K / 4; e += blockDim.x) {
+      const int r = (4 * e) / K, col = (4 * e) % K;
+      *reinterpret_cast<uint2*>(&wtr[kROff[L - 1] + r * ld + col]) =
+          *reinterpret_cast<const uint2*>(N.Wr[L] + (size_t)r * K + col);
+    }
+  }
+  v4f dw[kDwPerWave];
+#pragma unroll
+  for (int j = 0; j < kDwPerWave; j++) dw[j] = v4f{0, 0, 0, 0};
+  float dbias = 0.0f, wloss = 0.0f;
+  uint16_t* const tl = tiles[wave];
+  float* const z = zt[wave];
+  __syncthreads();
+  for (uint32_t it = blockIdx.x; it < nIter; it += gridDim.x) {
+    const uint32_t r0 = (it * kTrainWaves + wave) * 16;
+    const uint32_t gr = min(r0 + (uint32_t)lr, M ? M - 1 : 0u);
+    // ---- forward, transposed, in registers
+    v4s in[6];
+    {
+      const float4 p4 = T.pos[gr], d4 = T.dir[gr];
+      const Vec3 x = v3(p4.x, p4.y, p4.z), dd = v3(d4.x, d4.y, d4.z);
+#pragma unroll
+      for (int ks = 0; ks < 6; ks++) {
+        const uint32_t p0 = guide_pair(N, x, dd, 8 * ks + 2 * lg), p1 = guide_pair(N, x, dd, 8 * ks + 2 * lg + 1);
+        in[ks] = v4s{(short)(p0 & 0xffffu), (short)(p0 >> 16), (short)(p1 & 0xffffu), (short)(p1 >> 16)};
+      }
+    }
+    v4f acc[4];
+    v4s h1[4], h2[4], h3[4];
+    fwd_layer<6, 4>(wts + kWOff[0], kGIn + kWPad, N.b[0], in, acc, lr, lg);
+#pragma unroll
+    for (int k = 0; k < 4; k++) h1[k] = relu_bf16(acc[k]);
+    fwd_layer<4, 4>(wts + kWOff[1], kGHid + kWPad, N.b[1], h1, acc, lr, lg);
+#pragma unroll
+    for (int k = 0; k < 4; k++) h2[k] = relu_bf16(acc[k]);
+    fwd_layer<4, 4>(wts + kWOff[2], kGHid + kWPad, N.b[2], h2, acc, lr, lg);
+#pragma unroll
+    for (int k = 0; k < 4; k++) h3[k] = relu_bf16(acc[k]);
+    fwd_layer<4, 3>(wts + kWOff[3], kGHid + kWPad, N.b[3], h3, acc, lr, lg);
+#pragma unroll
+    for (int mt = 0; mt < 3; mt++)
+      *reinterpret_cast<float4*>(z + lr * kGOut + mt * 16 + 4 * lg) = make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
+#pragma unroll
+    for (int ks = 0; ks < 6; ks++) store_frag(tl, kFX + 16 * ks, in[ks], lr, lg);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      store_frag(tl, kFH1 + 16 * k, h1[k], lr, lg);
+      store_frag(tl, kFH2 + 16 * k, h2[k], lr, lg);
+      store_frag(tl, kFH3 + 16 * k, h3[k], lr, lg);
+    }
+    wave_lds_sync();
+    // ---- loss -wt log(alpha q + (1 - alpha) p_bsdf) and its gradient: four lanes per row, two lobes each
+    {
+      const int r = lane >> 2, sub = lane & 3;
+      const uint32_t row = r0 + (uint32_t)r;
+      const bool valid = row < M;
+      const uint32_t rr = valid ? row : 0u;
+      const float* zr = z + r * kGOut;
+      const float4 w4 = T.wdir[rr];
+      const float om[3] = {w4.x, w4.y, w4.z};
+      const float pbr = T.pb[rr], wr = valid ? T.wt[rr] : 0.0f;
+      float pi[2], v[2], mu[2][3], ml[2], kap[2], cj[2];
+      bool clamped[2];
+      float mx = fmaxf(zr[10 * sub], zr[10 * sub + 5]);
+      mx = fmaxf(mx, __shfl_xor(mx, 1));
+      mx = fmaxf(mx, __shfl_xor(mx, 2));
+      float qs = 0.0f, es = 0.0f;
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int j = 2 * sub + u;
+        pi[u] = expf(zr[5 * j] - mx);
+        es += pi[u];
+        const float m0 = zr[5 * j + 1], m1 = zr[5 * j + 2], m2 = zr[5 * j + 3];
+        const float l = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
+        ml[u] = l;
+        mu[u][0] = m0 / l;
+        mu[u][1] = m1 / l;
+        mu[u][2] = m2 / l;
+        const float lk = zr[5 * j + 4];
+        clamped[u] = lk < -3.0f || lk > 8.0f;
+        kap[u] = expf(fminf(8.0f, fmaxf(-3.0f, lk)));
+        cj[u] = mu[u][0] * om[0] + mu[u][1] * om[1] + mu[u][2] * om[2];
+        v[u] = kap[u] / (6.28318530717958647692f * (1.0f - expf(-2.0f * kap[u]))) * expf(kap[u] * (cj[u] - 1.0f));
+      }
+      es += __shfl_xor(es, 1);
+      es += __shfl_xor(es, 2);
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        pi[u] /= es;
+        qs += pi[u] * v[u];
+      }
+      qs += __shfl_xor(qs, 1);
+      qs += __shfl_xor(qs, 2);
+      const float q = qs;
+      const float sig = 1.0f / (1.0f + expf(-zr[40]));
+      const float alpha = kAlphaMin + kAlphaSpan * sig;
+      const float pm = fmaxf(alpha * q + (1.0f - alpha) * pbr, 1e-12f);
+      const float gp = -wr / pm, dq = gp * alpha;
+      uint16_t* dz = tl + kFDZ * 16;  // feature-major dZ^T
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int j = 2 * sub + u;
+        const float dv = dq * pi[u], k = kap[u];
+        const float e = expf(-2.0f * k);
+        const float dlk = 1.0f / k - 2.0f * e / (1.0f - e) + (cj[u] - 1.0f);
+        const float sc = dv * v[u] * k / ml[u];
+        dz[(5 * j) * 16 + r] = f2bf(dq * pi[u] * (v[u] - q));
+        for (int a = 0; a < 3; a++) dz[(5 * j + 1 + a) * 16 + r] = f2bf(sc * (om[a] - cj[u] * mu[u][a]));
+        dz[(5 * j + 4) * 16 + r] = f2bf(clamped[u] ? 0.0f : dv * v[u] * dlk * k);
+      }
+      if (sub == 0) {
+        dz[40 * 16 + r] = f2bf(gp * (q - pbr) * kAlphaSpan * sig * (1.0f - sig));
+        wloss += valid ? -wr * logf(pm) : 0.0f;
+      } else if (sub == 1) {
+        for (int o = 41; o < kGOut; o++) dz[o * 16 + r] = 0;
+      }
+    }
+    wave_lds_sync();
+    // ---- backward deltas, transposed: D_l^T = (W_{l+1}^T D_{l+1}^T) * (H_l > 0)
+    v4s dzb[3];
+#pragma unroll
+    for (int ks = 0; ks < 3; ks++)
+#pragma unroll
+      for (int s2 = 0; s2 < 4; s2++) dzb[ks][s2] = (short)tl[(kFDZ + 16 * ks + 4 * lg + s2) * 16 + lr];
+    v4s dh[4];
+    {
+      const uint16_t* w4 = wtr + kROff[2];
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        v4f a = v4f{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < 3; ks++) a = mfma(ld4(w4 + (mt * 16 + lr) * (kGOut + kRPad) + ks * 16 + 4 * lg), dzb[ks], a);
+#pragma unroll
+        for (int i = 0; i < 4; i++) dh[mt][i] = (short)f2bf(bf2f((uint16_t)h3[mt][i]) > 0.0f ? a[i] : 0.0f);
+        store_frag(tl, kFD3 + 16 * mt, dh[mt], lr, lg);
+      }
+    }
+#pragma unroll
+    for (int step = 0; step < 2; step++) {  // D2 from D3 (W3), D1 from D2 (W2)
+      const uint16_t* wl = wtr + kROff[1 - step];
+      const v4s* hm = step == 0 ? h2 : h1;
+      v4s nd[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        v4f a = v4f{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) a = mfma(ld4(wl + (mt * 16 + lr) * (kGHid + kRPad) + ks * 16 + 4 * lg), dh[ks], a);
+#pragma unroll
+        for (int i = 0; i < 4; i++) nd[mt][i] = (short)f2bf(bf2f((uint16_t)hm[mt][i]) > 0.0f ? a[i] : 0.0f);
+        store_frag(tl, (step == 0 ? kFD2 : kFD1) + 16 * mt, nd[mt], lr, lg);
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) dh[mt] = nd[mt];
+    }
+    __syncthreads();
+    // ---- weight gradients: dW_l[out][in] += sum over the block's 64 rows of D_l^T[out][row] H_{l-1}^T[in][row]
+#pragma unroll
+    for (int j = 0; j < kDwPerWave; j++) {
+      int L, mo, mi;
+      dw_tile(wave * kDwPerWave + j, L, mo, mi);
+      const int fD = L == 0 ? kFD1 : L == 1 ? kFD2 : L == 2 ? kFD3 : kFDZ;
+      const int fH = L == 0 ? kFX : L == 1 ? kFH1 : L == 2 ? kFH2 : kFH3;
+      v4f a = dw[j];
+#pragma unroll
+      for (int wv = 0; wv < kTrainWaves; wv++) {
+        const uint16_t* tw = tiles[wv];
+        a = mfma(ld4(tw + (fD + 16 * mo + lr) * 16 + 4 * lg), ld4(tw + (fH + 16 * mi + lr) * 16 + 4 * lg), a);
+      }
+      dw[j] = a;
+    }
+    if (threadIdx.x < 240) {  // bias gradients: feature threadIdx.x of D1 | D2 | D3 | dZ
+      const int t = threadIdx.x;
+      const int f = t < 64 ? kFD1 + t : t < 128 ? kFD2 + t - 64 : t < 192 ? kFD3 + t - 128 : kFDZ + t - 192;
+      for (int wv = 0; wv < kTrainWaves; wv++)
+        for (int r = 0; r < 16; r++) dbias += bf2f(tiles[wv][f * 16 + r]);
+    }
+    __syncthreads();
+  }
+  // ---- this block's partial: weights (D layout: out 16 mo + 4 lg + i, in 16 mi + lr), biases, loss
+  float* out = partial + (size_t)blockIdx.x * kGParams;
+#pragma unroll
+  for (int j = 0; j < kDwPerWave; j++) {
+    int L, mo, mi;
+    dw_tile(wave * kDwPerWave + j, L, mo, mi);
+    const int OUT = kLayerOut[L];
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[kParamBase[L] + (16 * mi + lr) * OUT + 16 * mo + 4 * lg + i] = dw[j][i];
+  }
+  if (threadIdx.x < 240) out[kBiasBase + threadIdx.x] = dbias;
+  for (int o = 1; o < 64; o <<= 1) wloss += __shfl_xor(wloss, o);
+  if (lane == 0) lsum[wave] = wloss;
+  __syncthreads();
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = lsum[0] + lsum[1] + lsum[2] + lsum[3];
+}
+
+// Mean gradient of a step from the blocks' partials (added in block order),
+// then the Adam update of the FP32 masters and their BF16 copies.
+__global__ void k_adam_reduce(AdamArgs A, const float* partial, uint32_t nblocks) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= kGParams) return;
   int L, off;
   bool isb;
   param_loc(idx, L, off, isb);
-  const int IN = kLayerIn[L], OUT = kLayerOut[L], P = IN * OUT + OUT;
-  const int poff = isb ? IN * OUT + off : off;
+  const int IN = kLayerIn[L], OUT = kLayerOut[L];
   const uint32_t M = min(*A.rows, A.cap);
-  const uint32_t nch = (M + kDwChunk - 1) / kDwChunk;
   float g = 0.0f;
-  for (uint32_t c = 0; c < nch; c++) g += A.partial[L][(size_t)c * P + poff];
+  for (uint32_t b = 0; b < nblocks; b++) g += partial[(size_t)b * kGParams + idx];
   g = M ? g / (float)M : 0.0f;
   if (A.grad_out) A.grad_out[idx] = g;
   float m = A.b1 * A.m[idx] + (1.0f - A.b1) * g;
@@ -485,24 +502,6 @@ __global__ void k_guide_sync(AdamArgs A) {
   A.Wt[L][j * IN + i] = h;
 }
 
-// Encode queries given as positions and directions (test and training path).
-__global__ void k_guide_encode(GuideNet N, const float4* pos, const float4* dir, const uint32_t* rows, uint32_t cap,
-                               uint16_t* X) {
-  const uint32_t M = min(*rows, cap);
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < M; r += gridDim.x * blockDim.x) {
-    float4 p = pos[r], d = dir[r];
-    guide_encode(N, v3(p.x, p.y, p.z), v3(d.x, d.y, d.z), X + (size_t)r * kGIn);
-  }
-}
-
-hipError_t guide_forward(const GuideNet& N, const uint16_t* X, const uint32_t* rows, uint32_t cap, uint16_t* A1,
-                         uint16_t* A2, uint16_t* A3, float* Z4, int n_cu, hipStream_t s) {
-  const uint32_t tiles = (cap + 15) / 16;
-  const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + 3) / 4, (uint32_t)n_cu * 8));
-  hipLaunchKernelGGL(k_mlp_fwd, dim3(blocks), dim3(256), 0, s, N, X, rows, cap, A1, A2, A3, Z4);
-  return hipGetLastError();
-}
-
 // ------------------------------------------------------------------ state
 static AdamArgs adam_args(GuideState& G) {
   AdamArgs A{};
@@ -512,37 +511,28 @@ static AdamArgs adam_args(GuideState& G) {
   for (int l = 0; l < 4; l++) {
     A.Wt[l] = G.Wt[l].as<uint16_t>();
     A.Wr[l] = G.Wr[l].as<uint16_t>();
-    A.partial[l] = G.partial[l].as<float>();
   }
   return A;
 }
 
 int guide_reserve(rx_ctx* c, GuideState& G, uint32_t n) {
-  if (n <= G.cap) return RX_OK;
-  const size_t N = n;
   hipError_t e = hipSuccess;
   auto E = [&](DBuf& b, size_t bytes) {
     if (e == hipSuccess) e = b.ensure(bytes);
   };
-  E(G.X, N * kGIn * 2);
-  E(G.A1, N * kGHid * 2);
-  E(G.A2, N * kGHid * 2);
-  E(G.A3, N * kGHid * 2);
-  E(G.Z4, N * kGOut * 4);
-  E(G.dZ4, N * kGOut * 2);
-  E(G.D1, N * kGHid * 2);
-  E(G.D2, N * kGHid * 2);
-  E(G.D3, N * kGHid * 2);
-  const size_t nch = (N + kDwChunk - 1) / kDwChunk;
-  for (int l = 0; l < 4; l++) E(G.partial[l], nch * (kLayerIn[l] * kLayerOut[l] + kLayerOut[l]) * 4);
-  E(G.loss_part, ((N + 255) / 256) * 4);
-  E(G.pos, N * 16);
-  E(G.dir, N * 16);
-  E(G.wdir, N * 16);
-  E(G.pb, N * 4);
-  E(G.wt, N * 4);
+  E(G.tpart, (size_t)c->n_cu * kGParams * 4);  // one partial per training block (at most one per CU)
+  E(G.loss_part, (size_t)c->n_cu * 4);
+  if (n > G.cap) {
+    const size_t N = n;
+    E(G.Z4, N * kGOut * 4);
+    E(G.pos, N * 16);
+    E(G.dir, N * 16);
+    E(G.wdir, N * 16);
+    E(G.pb, N * 4);
+    E(G.wt, N * 4);
+  }
   if (int r = hip_err(c, e, "guide buffers")) return r;
-  G.cap = n;
+  G.cap = std::max(G.cap, n);
   return RX_OK;
 }
 
@@ -558,37 +548,21 @@ int guide_reserve_store(rx_ctx* c, GuideState& G, uint32_t n) {
   return RX_OK;
 }
 
-// One training step on rows already in G.X / G.wdir / G.pb / G.wt with the
-// row count at G.rows (device).  Loss partials stay on the device.
+// One training step (k_guide_train + k_adam_reduce) on the rows in G.pos /
+// G.dir / G.wdir / G.pb / G.wt with the row count at G.rows (device).  Loss
+// partials stay on the device.
 static int guide_train_device(rx_ctx* c, GuideState& G, uint32_t cap, float* grad_out) {
   hipStream_t s = c->stream;
-  GuideNet N = G.net();
   const uint32_t* rows = G.rows.as<uint32_t>();
-  if (int r = hip_err(c, guide_forward(N, G.X.as<uint16_t>(), rows, cap, G.A1.as<uint16_t>(), G.A2.as<uint16_t>(),
-                                       G.A3.as<uint16_t>(), G.Z4.as<float>(), c->n_cu, s),
-                      "guide forward"))
-    return r;
-  hipLaunchKernelGGL(k_guide_loss, dim3((cap + 255) / 256), dim3(256), 0, s, G.Z4.as<float>(), G.wdir.as<float4>(),
-                     G.pb.as<float>(), G.wt.as<float>(), rows, cap, G.dZ4.as<uint16_t>(), G.loss_part.as<float>());
-  const uint32_t tiles = (cap + 15) / 16;
-  const int bb = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + 3) / 4, (uint32_t)c->n_cu * 8));
-  hipLaunchKernelGGL(k_mlp_bwd, dim3(bb), dim3(256), 0, s, N, G.dZ4.as<uint16_t>(), rows, cap, G.A1.as<uint16_t>(),
-                     G.A2.as<uint16_t>(), G.A3.as<uint16_t>(), G.D1.as<uint16_t>(), G.D2.as<uint16_t>(),
-                     G.D3.as<uint16_t>());
-  const uint32_t nch = (cap + kDwChunk - 1) / kDwChunk;
-  const int db = (int)std::max<uint32_t>(1, (nch + 3) / 4);
-  hipLaunchKernelGGL((k_mlp_dw<kGIn, kGHid>), dim3(db), dim3(256), 0, s, G.X.as<uint16_t>(), G.D1.as<uint16_t>(),
-                     rows, cap, G.partial[0].as<float>());
-  hipLaunchKernelGGL((k_mlp_dw<kGHid, kGHid>), dim3(db), dim3(256), 0, s, G.A1.as<uint16_t>(), G.D2.as<uint16_t>(),
-                     rows, cap, G.partial[1].as<float>());
-  hipLaunchKernelGGL((k_mlp_dw<kGHid, kGHid>), dim3(db), dim3(256), 0, s, G.A2.as<uint16_t>(), G.D3.as<uint16_t>(),
-                     rows, cap, G.partial[2].as<float>());
-  hipLaunchKernelGGL((k_mlp_dw<kGHid, kGOut>), dim3(db), dim3(256), 0, s, G.A3.as<uint16_t>(), G.dZ4.as<uint16_t>(),
-                     rows, cap, G.partial[3].as<float>());
+  const uint32_t iters = (cap + 16 * kTrainWaves - 1) / (16 * kTrainWaves);
+  const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(iters, (uint32_t)c->n_cu));
+  const TrainRows T{G.pos.as<float4>(), G.dir.as<float4>(), G.wdir.as<float4>(), G.pb.as<float>(), G.wt.as<float>()};
+  hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(kTrainBlock), 0, s, G.net(), T, rows, cap, G.tpart.as<float>(),
+                     G.loss_part.as<float>());
+  G.train_blocks = blocks;
   G.step++;
   AdamArgs A = adam_args(G);
   A.grad_out = grad_out;
-  A.nchunks = nch;
   A.rows = rows;
   A.cap = cap;
   A.lr = G.prm.learning_rate;
@@ -599,7 +573,7 @@ static int guide_train_device(rx_ctx* c, GuideState& G, uint32_t cap, float* gra
   A.eps = G.prm.eps;
   A.c1 = 1.0f - powf(A.b1, (float)G.step);
   A.c2 = 1.0f - powf(A.b2, (float)G.step);
-  hipLaunchKernelGGL(k_adam, dim3((kGParams + 255) / 256), dim3(256), 0, s, A);
+  hipLaunchKernelGGL(k_adam_reduce, dim3((kGParams + 255) / 256), dim3(256), 0, s, A, G.tpart.as<float>(), blocks);
   return hip_err(c, hipGetLastError(), "guide train launch");
 }
 
@@ -628,13 +602,7 @@ static void init_params(std::vector<float>& p, uint32_t seed) {
 
 // Train on rows the renderer already placed in G.pos/G.dir/G.wdir/G.pb/G.wt
 // with the row count in G.rows (device); capacity `cap` was reserved.
-int guide_train_rows(rx_ctx* c, uint32_t cap) {
-  GuideState& G = *c->guide;
-  hipLaunchKernelGGL(k_guide_encode, dim3(std::min<uint32_t>((cap + 255) / 256, 4096u)), dim3(256), 0, c->stream,
-                     G.net(), G.pos.as<float4>(), G.dir.as<float4>(), G.rows.as<uint32_t>(), cap, G.X.as<uint16_t>());
-  if (int r = hip_err(c, hipGetLastError(), "encode")) return r;
-  return guide_train_device(c, G, cap, nullptr);
-}
+int guide_train_rows(rx_ctx* c, uint32_t cap) { return guide_train_device(c, *c->guide, cap, nullptr); }
 
 }  // namespace rx
 
@@ -723,10 +691,7 @@ static int upload_queries(rx_ctx* c, GuideState& G, uint32_t n, const float* pos
   hipStream_t s = c->stream;
   if (int r = hip_err(c, G.pos.upload(p4.data(), p4.size() * 4, s), "upload pos")) return r;
   if (int r = hip_err(c, G.dir.upload(d4.data(), d4.size() * 4, s), "upload dir")) return r;
-  if (int r = hip_err(c, G.rows.upload(&n, 4, s), "upload rows")) return r;
-  hipLaunchKernelGGL(k_guide_encode, dim3(std::min<uint32_t>((n + 255) / 256, 4096u)), dim3(256), 0, s, G.net(),
-                     G.pos.as<float4>(), G.dir.as<float4>(), G.rows.as<uint32_t>(), n, G.X.as<uint16_t>());
-  return hip_err(c, hipGetLastError(), "encode");
+  return hip_err(c, G.rows.upload(&n, 4, s), "upload rows");
 }
 
 int rx_guide_eval(rx_ctx* c, uint32_t n, const float* pos, const float* dir, float* out48) {
@@ -763,7 +728,7 @@ int rx_guide_train_batch(rx_ctx* c, uint32_t n, const float* pos, const float* d
   if (int r = hip_err(c, G.wt.upload(weight, 4 * (size_t)n, s), "upload weight")) return r;
   if (int r = guide_train_device(c, G, n, grad ? G.grad.as<float>() : nullptr)) return r;
   if (loss) {
-    std::vector<float> lp((n + 255) / 256);
+    std::vector<float> lp(G.train_blocks);
     if (int r = hip_err(c, hipMemcpyAsync(lp.data(), G.loss_part.p, lp.size() * 4, hipMemcpyDeviceToHost, s), "loss"))
       return r;
     if (int r = hip_err(c, hipStreamSynchronize(s), "loss sync")) return r;

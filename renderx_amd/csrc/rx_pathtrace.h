@@ -53,7 +53,10 @@ constexpr uint32_t FLAG_MIS = 1u;     // last scatter was RANDOM (only transpare
 // 1 / (1 - alpha) where the mixture misses, and over a long random walk in a
 // medium such factors compound into heavy-tailed throughputs.
 constexpr uint32_t kGuidedShift = 8;
-constexpr uint32_t kMaxGuided = 3;
+#ifndef RX_GUIDE_MAX_VERTICES
+#define RX_GUIDE_MAX_VERTICES 3
+#endif
+constexpr uint32_t kMaxGuided = RX_GUIDE_MAX_VERTICES;
 
 struct DevObject {
   int32_t type, vol;
@@ -606,8 +609,9 @@ __device__ __forceinline__ float guided_rr(const DevObject& O) {
 
 // Guided surface scatter: with probability alpha sample the learned vMF
 // mixture, otherwise the reference's BSDF strategy; weight f cos / p_mix with
-// p_mix = alpha q + (1 - alpha) p_bsdf (one-sample MIS, unbiased because
-// alpha <= 0.95 keeps the BSDF density in the mixture).
+// p_mix = alpha q + (1 - alpha) p_bsdf (one-sample MIS: unbiased because
+// alpha < 1 keeps the BSDF density in the mixture, and alpha <= 0.5 bounds the
+// weight by twice the BSDF strategy's).
 __device__ void guided_scatter(const DevObject& O, int obj, Vec3 x, Vec3 d, Vec3 n, int hitPos, int insideIn,
                                bool russian, const GuideHead& H, const float* ug, float u_rr, Scat& sc) {
   sc.x = x;
@@ -629,6 +633,14 @@ __device__ void guided_scatter(const DevObject& O, int obj, Vec3 x, Vec3 d, Vec3
   Vec3 f;
   float pb;
   bsdf_nrr(O, n, d, w, f, pb);
+  // The continuation multiplies by the cosine term clamped at 1e-6 (the
+  // reference's COS_TERM_MIN); for directions closer to the tangent plane
+  // the true cosine is folded into f, so the weight stays f |cos| / p_mix:
+  // the BSDF density in p_mix vanishes with the cosine, the clamped term
+  // would not, and a guide-proposed grazing direction would get an
+  // unbounded weight.
+  const float cn = fabsf(dot(n, w));
+  if (cn < 1e-6f) f = f * (cn * 1e6f);
   const float q = guide_pdf(H, w);
   const float pmix = H.alpha * q + (1.0f - H.alpha) * pb;
   const float qrr = russian ? guided_rr(O) : 1.0f;
@@ -1270,15 +1282,32 @@ static __global__ void k_rec_count(const float* rec, uint32_t K, float tag, uint
   if (threadIdx.x == 0) bcount[blockIdx.x] = red[0];
 }
 
-static __global__ void k_rec_scan(uint32_t* bcount, uint32_t nb, uint32_t* total) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Exclusive scan of the chunk counts in place, one block of kScanThreads:
+// each thread sums a contiguous run of counts, the block scans the run sums,
+// and each thread writes its run's offsets.
+constexpr int kScanThreads = 1024;
+static __global__ void __launch_bounds__(kScanThreads) k_rec_scan(uint32_t* bcount, uint32_t nb, uint32_t* total) {
+  __shared__ uint32_t part[kScanThreads];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
+  const uint32_t lo = min(nb, t * per), hi = min(nb, lo + per);
   uint32_t s = 0;
-  for (uint32_t i = 0; i < nb; i++) {
-    const uint32_t c = bcount[i];
-    bcount[i] = s;
-    s += c;
+  for (uint32_t i = lo; i < hi; i++) s += bcount[i];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < kScanThreads; o <<= 1) {  // inclusive scan of the run sums (Hillis-Steele)
+    const uint32_t v = t >= (uint32_t)o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
   }
-  *total = s;
+  uint32_t run = part[t] - s;
+  for (uint32_t i = lo; i < hi; i++) {
+    const uint32_t c = bcount[i];
+    bcount[i] = run;
+    run += c;
+  }
+  if (t == kScanThreads - 1) *total = part[t];
 }
 
 struct TrainOut {

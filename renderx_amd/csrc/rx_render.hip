@@ -258,39 +258,46 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   const int mode = (int)prm->mode;
   GuideArgs GA{};
   const BatchArgs BA0{};
-  uint32_t train_passes = 0, batch = 0, steps = 0, store_cap = 0;
+  uint32_t train_passes = 0, batch = 0, steps = 0, store_cap = 0, s_train = 1, train_end = 0;
   if (mode == RX_MODE_GUIDED) {
     if (!c->guide) return set_err(c, RX_ERR_STATE, "rx_render: guided mode needs rx_guide_init");
-    // a training pass keeps one record per pixel and bounce, ranked with 32-bit indices
-    if ((uint64_t)npix * (prm->max_depth + 1) >= (1ull << 31))
-      return set_err(c, RX_ERR_INVALID, "rx_render: guided mode needs pixels * (max_depth + 1) < 2^31");
     GuideState& G = *c->guide;
     for (int a = 0; a < 3; a++) {  // encode positions relative to the scene box (slightly padded)
       float pad = 1e-3f * (R.box_hi[a] - R.box_lo[a]) + 1e-6f;
       G.lo[a] = R.box_lo[a] - pad;
       G.hi[a] = R.box_hi[a] + pad;
     }
-    // samples with index < train_passes train the network online (a render
-    // resumed with sample_offset >= train_passes only uses it)
+    // The online warm-up: train_passes passes of s_train samples per pixel
+    // each, the leading train_end samples of every pixel.  A pass has enough
+    // paths for its sample store (batch * steps rows, an evenly strided
+    // subset of the pass's records) and at least kTrainPassPaths paths: a
+    // pass with fewer waits on its longest paths at every bounce.  A render
+    // resumed with sample_offset >= train_end only uses the network.
+    constexpr uint32_t kTrainPassPaths = 1u << 20;
     train_passes = G.prm.train_passes;
     batch = std::max(256u, G.prm.batch_size);
     steps = std::max(1u, G.prm.steps_per_pass);
     store_cap = (uint32_t)std::min<uint64_t>((uint64_t)batch * steps, 1u << 26);
+    s_train = std::max((store_cap + npix - 1) / npix, (kTrainPassPaths + npix - 1) / npix);
+    s_train = std::max(1u, std::min(S_max, s_train));
+    train_end = train_passes * s_train;
     G.decay_steps = train_passes * steps;
-    if (int r = guide_reserve(c, G, batch)) return r;  // the training step's rows
-    if (prm->sample_offset < train_passes && prm->spp)
-      if (int r = guide_reserve_store(c, G, store_cap)) return r;
     const uint32_t Rs = prm->max_depth + 1;
-    if (prm->sample_offset < train_passes && prm->spp) {
-      // a training pass renders one sample per pixel: npix paths
-      RXCK(R.rec.ensure((size_t)npix * Rs * kRecFloats * 4), "alloc records");
-      RXCK(R.bcount.ensure(((size_t)npix * Rs / kRecChunk + 2) * 4), "alloc records");
+    // a training pass keeps one record per path slot and bounce, ranked with 32-bit indices
+    if ((uint64_t)npix * s_train * Rs >= (1ull << 31))
+      return set_err(c, RX_ERR_INVALID, "rx_render: guided mode needs pixels * (max_depth + 1) < 2^31");
+    if (int r = guide_reserve(c, G, batch)) return r;  // the training step's rows
+    if (prm->sample_offset < train_end && prm->spp) {
+      if (int r = guide_reserve_store(c, G, store_cap)) return r;
+      const size_t K = (size_t)npix * s_train * Rs;
+      RXCK(R.rec.ensure(K * kRecFloats * 4), "alloc records");
+      RXCK(R.bcount.ensure((K / kRecChunk + 2) * 4), "alloc records");
       RXCK(R.total.ensure(64), "alloc records");
       // tags restart at 1 every render: clear stale records once
-      hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, R.rec.as<double>(), (size_t)npix * Rs * kRecFloats / 2);
+      hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, R.rec.as<double>(), K * kRecFloats / 2);
     }
     GA.net = G.net();
-    RXCK(R.ghead.ensure(kGHead * (size_t)N), "alloc guide heads");      // 144 B per path slot
+    RXCK(R.ghead.ensure(kGHead * (size_t)N), "alloc guide heads");      // 208 B per path slot
     RXCK(R.gquery.ensure(32 * (size_t)N), "alloc guide queries");       // 32 B per path slot
     RXCK(R.grow.ensure(4 * (size_t)N), "alloc guide rows");
     GA.head = R.ghead.as<unsigned char>();
@@ -317,9 +324,11 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   uint32_t pass = 0;
   for (uint32_t done = 0; done < prm->spp; pass++) {
     P.sample = prm->sample_offset + done;
-    // a training pass renders one sample per pixel; otherwise up to S_max samples per pass
-    const bool train = mode == RX_MODE_GUIDED && P.sample < train_passes;
-    const uint32_t spass = train ? 1u : std::min(S_max, prm->spp - done);
+    // a training pass renders up to s_train samples per pixel (to the next pass boundary); otherwise up to
+    // S_max samples per pass
+    const bool train = mode == RX_MODE_GUIDED && P.sample < train_end;
+    const uint32_t spass = train ? std::min(s_train - P.sample % s_train, prm->spp - done)
+                                 : std::min(S_max, prm->spp - done);
     P.n_paths = npix * spass;
     P.spass = spass;
     done += spass;
@@ -372,7 +381,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       const uint32_t K = P.n_paths * GA.R;
       const uint32_t nb = (K + kRecChunk - 1) / kRecChunk;
       hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(256), 0, s, GA.rec, K, GA.tag, R.bcount.as<uint32_t>());
-      hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(64), 0, s, R.bcount.as<uint32_t>(), nb, R.total.as<uint32_t>());
+      hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(kScanThreads), 0, s, R.bcount.as<uint32_t>(), nb,
+                         R.total.as<uint32_t>());
       const TrainOut ST{G.spos.as<float4>(), G.sdir.as<float4>(), G.swdir.as<float4>(), G.spb.as<float>(),
                         G.swt.as<float>(), G.srows.as<uint32_t>()};
       const TrainOut TB{G.pos.as<float4>(), G.dir.as<float4>(), G.wdir.as<float4>(), G.pb.as<float>(),
