@@ -138,6 +138,45 @@ def equal_time_relmse(res=512, ref_spp=65536, budget_s=5.0, chunk=64):
     return out
 
 
+# FP32-accumulated BF16 MFMA work per network row (96 -> 64 -> 64 -> 64 -> 41):
+# the forward pass, and forward + input gradients of layers 2-4 + weight
+# gradients of all four layers in training
+GUIDE_FWD_FLOP = 2 * (96 * 64 + 64 * 64 + 64 * 64 + 64 * 41)
+GUIDE_TRAIN_FLOP = GUIDE_FWD_FLOP + 2 * (64 * 41 + 64 * 64 + 64 * 64) + GUIDE_FWD_FLOP
+
+
+def guided_profile(res=512, calls=4, chunk=64):
+    """One guided render of the media + glass scene as the equal-time study
+    runs it (8 warm-up passes of 2^18 training samples, then `calls` render
+    calls of `chunk` samples per pixel), and the same samples NEE-only.  The
+    network rows inferred and trained give the MFMA kernels' FLOP counts
+    (GUIDE_FWD_FLOP / GUIDE_TRAIN_FLOP per row) for a kernel trace of this
+    run."""
+    from renderx_amd import api, scenes
+
+    sc = scenes.media_scene(res, res, max_depth=16, freq=158, glass_freq=16)
+    ctx = api.Context(0)
+    out = {"scene": f"media+glass+Phong sphere {sc.n_tris} tris, {res}x{res} px, max depth 16",
+           "calls": calls, "spp_per_call": chunk, "fwd_flop_per_row": GUIDE_FWD_FLOP,
+           "train_flop_per_row": GUIDE_TRAIN_FLOP}
+    for name, mode in [("guided", api.MODE_GUIDED), ("nee", api.MODE_NEE)]:
+        pt = api.PathTracerHIP(sc, ctx=ctx)
+        if mode == api.MODE_GUIDED:
+            pt.guide_init(seed=7, train_passes=8, batch_size=16384, steps_per_pass=16, learning_rate=1e-2)
+        rows = trows = 0
+        ms = []
+        for k in range(calls):
+            r = pt.renderPixels(spp=chunk, mode=mode, seed=5, sample_offset=k * chunk)
+            st = r["stats"]
+            rows += int(st["guide_rows"])
+            trows += int(st["guide_train_rows"])
+            ms.append(float(st["ms_total"]))
+            log(f"[bench] guided profile {name} call {k + 1}/{calls}: {ms[-1]:.1f} ms, {int(st['guide_rows'])} "
+                f"network rows, {int(st['guide_train_rows'])} training rows")
+        out[name] = {"ms_per_call": ms, "guide_rows": rows, "guide_train_rows": trows}
+    return out
+
+
 def batch64(steps=2):
     """The 64-scene triangle-soup batch (10^4..2*10^6 triangles per scene,
     128^2..1024^2 pixels, 4..64 spp, max depth 8): all scenes rendered as one
@@ -219,10 +258,15 @@ def main():
     ap.add_argument("--no-bpt", action="store_true", help="skip the bidirectional tracer")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (reference kernel for PMC byte counters)")
+    ap.add_argument("--guided-profile", action="store_true",
+                    help="only the guided render of the equal-time study (for a kernel trace); prints its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only"}))
         sys.exit(2)
+    if args.guided_profile:
+        print(json.dumps(guided_profile()), flush=True)
+        return
 
     from renderx_amd import api, scenes
 

@@ -102,7 +102,7 @@ enum rx_render_flags {
  * order and the random numbers are keyed by pixel and sample); only the time
  * changes.  Zero fields take the defaults. */
 typedef struct rx_render_tuning {
-  uint32_t path_slots;    /* path states resident at once (default 2^25, 2^22 in guided mode): a pass renders
+  uint32_t path_slots;    /* path states resident at once (default 2^25): a pass renders
                              min(spp, path_slots / pixels) samples of every pixel (>= 1) */
   uint32_t blocks_per_cu; /* resident 128-thread blocks per CU of the render kernels
                              (default and maximum: the traversal kernel's occupancy limit) */
@@ -123,6 +123,8 @@ typedef struct rx_render_stats {
   double ms_shade;
   double ms_shadow;
   uint64_t extend_launches;
+  uint64_t guide_rows;       /* guided mode: network queries evaluated (one per guided vertex) */
+  uint64_t guide_train_rows; /* guided mode: training rows over all Adam steps (batch_size per step) */
 } rx_render_stats;
 
 typedef struct rx_ctx rx_ctx;

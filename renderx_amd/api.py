@@ -24,7 +24,7 @@ RENDER_PARAMS = np.dtype([("spp", "<u4"), ("max_depth", "<u4"), ("rr_start_depth
 RENDER_STATS = np.dtype([("rays", "<u8"), ("extend_rays", "<u8"), ("shadow_rays", "<u8"), ("segments", "<u8"),
                          ("samples", "<u8"), ("box_tests", "<u8"), ("tri_tests", "<u8"), ("ms_total", "<f8"),
                          ("ms_extend", "<f8"), ("ms_shade", "<f8"), ("ms_shadow", "<f8"),
-                         ("extend_launches", "<u8")])
+                         ("extend_launches", "<u8"), ("guide_rows", "<u8"), ("guide_train_rows", "<u8")])
 RENDER_TIMING, RENDER_COUNT_WORK = 1, 2
 RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use_graph", "<u4"), ("reserved", "<u4")])
 GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("batch_size", "<u4"), ("steps_per_pass", "<u4"), ("learning_rate", "<f4"),

@@ -96,7 +96,7 @@ struct PathBufs {
   float4 *so, *sd, *sc;
   int2* ss;
   double *sum, *sumsq;
-  unsigned long long* stats;  // 0 rays, 1 extend rays, 2 shadow rays, 3 segments, 4 boxes, 5 tris
+  unsigned long long* stats;  // 0 rays, 1 extend rays, 2 shadow rays, 3 segments, 4 boxes, 5 tris, 6 guide rows
   uint2* spill;
   int32_t spill_depth;
   int32_t* overflow;
@@ -550,6 +550,7 @@ __global__ void __launch_bounds__(kRBlock) k_guide_query(DevScene S, PathBufs B,
     __syncthreads();
     staged_claim(qg, &B.cnt[kGuideRowBase + b]);
     __syncthreads();
+    if (threadIdx.x == 0 && counts[0]) atomicAdd(&B.stats[6], (unsigned long long)counts[0]);
     // the staged entries get rows base + k; their vertices are computed again (cached inputs)
     for (uint32_t k = threadIdx.x; k < counts[0]; k += blockDim.x) {
       const uint32_t i = list[k], r = counts[1] + k;

@@ -320,7 +320,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   RXCK(hipGetLastError(), "launch fill");
   RXCK(hipEventRecord(c->ev0, s), "event");
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ext, shd, sdw;
-  uint64_t launches = 0;
+  uint64_t launches = 0, train_rows = 0;
   uint32_t pass = 0;
   for (uint32_t done = 0; done < prm->spp; pass++) {
     P.sample = prm->sample_offset + done;
@@ -395,6 +395,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
                            R.total.as<uint32_t>(), store_cap, k, batch, TB);
         RXCK(hipGetLastError(), "launch gather");
         if (int r = guide_train_rows(c, batch)) return r;
+        train_rows += batch;
       }
     }
   }
@@ -424,6 +425,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     stats->samples = (uint64_t)npix * prm->spp;
     stats->ms_total = ms;
     stats->extend_launches = launches;
+    stats->guide_rows = st[6];
+    stats->guide_train_rows = train_rows;
     auto tsum = [](const std::vector<std::pair<hipEvent_t, hipEvent_t>>& v) {
       double t = 0;
       for (auto& e : v) {

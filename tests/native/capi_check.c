@@ -25,8 +25,8 @@ _Static_assert(sizeof(rx_render_params) == 32 && offsetof(rx_render_params, seed
                    offsetof(rx_render_params, flags) == 28,
                "rx_render_params layout");
 _Static_assert(sizeof(rx_render_tuning) == 16, "rx_render_tuning layout");
-_Static_assert(sizeof(rx_render_stats) == 96 && offsetof(rx_render_stats, ms_total) == 56 &&
-                   offsetof(rx_render_stats, extend_launches) == 88,
+_Static_assert(sizeof(rx_render_stats) == 112 && offsetof(rx_render_stats, ms_total) == 56 &&
+                   offsetof(rx_render_stats, extend_launches) == 88 && offsetof(rx_render_stats, guide_rows) == 96,
                "rx_render_stats layout");
 _Static_assert(sizeof(rx_guide_params) == 36, "rx_guide_params layout");
 
