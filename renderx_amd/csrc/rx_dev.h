@@ -118,6 +118,14 @@ __host__ __device__ __forceinline__ float slab_tnear(const float lo[3], const fl
 // Needs |d_a| >= 2^-30 on every axis (no overflow, no infinities); `beta`
 // holds 16u B_a / |d_a| + 2^-120 / |d_a|.  tnear receives slab_tnear's value.
 constexpr int kBoxUnknown = 0, kBoxMiss = 1, kBoxHit = 2;
+// median of three NaN-free floats (one v_med3_f32 on the device)
+__host__ __device__ __forceinline__ float med3f(float a, float b, float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_fmed3f(a, b, c);
+#else
+  return fmaxf(fminf(a, b), fminf(fmaxf(a, b), c));
+#endif
+}
 #ifndef RX_CERTAIN_HIT
 #define RX_CERTAIN_HIT 1  // 0: only the miss half (fewer registers in the traversal loop)
 #endif
@@ -145,18 +153,21 @@ __host__ __device__ __forceinline__ int slab_classify(const float lo[3], const f
   return miss ? kBoxMiss : kBoxUnknown;
 #endif
   // candidate face: the entry face if the entry is clearly ahead, else the
-  // exit face; its point must lie inside the other two slabs with the
-  // rounding bounds to spare (selects, no branches)
+  // exit face (the first axis whose entry, or exit, equals it); its point
+  // must lie inside the other two slabs with the rounding bounds to spare.
+  // Every candidate bound is computed before the selects, which keeps the
+  // loop free of branches.
   const float eux = fmaf(U8, fabsf(ex), ex) + beta.x, xdx = fmaf(-U8, fabsf(xx), xx) - beta.x;
   const float euy = fmaf(U8, fabsf(ey), ey) + beta.y, xdy = fmaf(-U8, fabsf(xy), xy) - beta.y;
   const float euz = fmaf(U8, fabsf(ez), ez) + beta.z, xdz = fmaf(-U8, fabsf(xz), xz) - beta.z;
   const bool entry = fmaf(-U8, fabsf(t), t) > 0.0f;
   const float tx = fminf(xx, fminf(xy, xz));
   const float tf = entry ? t : tx;
-  // the face's axis: the first axis whose entry (exit) equals tf
+  const float e_yz = fmaxf(euy, euz), e_xz = fmaxf(eux, euz), e_xy = fmaxf(eux, euy);
+  const float x_yz = fminf(xdy, xdz), x_xz = fminf(xdx, xdz), x_xy = fminf(xdx, xdy);
   const bool fx = tf == (entry ? ex : xx), fy = tf == (entry ? ey : xy);
-  const float eo = fx ? fmaxf(euy, euz) : (fy ? fmaxf(eux, euz) : fmaxf(eux, euy));
-  const float xo = fx ? fminf(xdy, xdz) : (fy ? fminf(xdx, xdz) : fminf(xdx, xdy));
+  const float eo = fx ? e_yz : (fy ? e_xz : e_xy);
+  const float xo = fx ? x_yz : (fy ? x_xz : x_xy);
   const bool ok = fmaf(-U8, fabsf(tf), tf) > 0.0f && eo <= tf * (1.0f - U24) && tf * (1.0f + U24) <= xo;
   return miss ? kBoxMiss : (ok ? kBoxHit : kBoxUnknown);
 }
