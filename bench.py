@@ -180,7 +180,11 @@ def guided_profile(res=512, calls=4, chunk=64):
 def batch64(steps=2):
     """The 64-scene triangle-soup batch (10^4..2*10^6 triangles per scene,
     128^2..1024^2 pixels, 4..64 spp, max depth 8): all scenes rendered as one
-    job with one recorded pass graph replayed per pass.  Returns rays/s."""
+    job.  Timed three ways: one recorded pass graph replayed per pass, the
+    whole batch as one persistent cooperative launch, and the shape the
+    autotuner (rx_batch_autotune: path slots, blocks per CU, extend stack
+    depth and block size, launch form) picks on the first 2 passes.  Returns
+    rays/s of the autotuned shape."""
     from renderx_amd import api, scenes
 
     t = time.perf_counter()
@@ -193,19 +197,37 @@ def batch64(steps=2):
         br.add(sc, mode=api.MODE_PT, seed=1000 + k)
     build_s = time.perf_counter() - t
     log(f"[bench] batch64: 64 trees built in {build_s:.1f} s")
+
+    def timed(n):
+        br.render()  # warm-up
+        rays = samples = 0
+        t0 = time.perf_counter()
+        for _ in range(n):
+            st = br.render()
+            rays += st["rays"]
+            samples += st["samples"]
+        return rays, samples, time.perf_counter() - t0
+
+    out = {"scenes": 64, "triangles": int(sum(s.n_tris for s in scs)), "host_tree_build_s": build_s}
     br.set_tuning(use_graph=1)
-    br.render()  # warm-up
-    rays = samples = 0
+    _, samples, wall = timed(steps)
+    out["graph_replay_ms_per_batch"] = wall / steps * 1e3
+    log(f"[bench] batch64 graph replay: {samples / steps / 1e6:.0f} M samples per batch, {wall / steps * 1e3:.0f} ms")
+    br.set_tuning(use_graph=2)
+    _, samples, wall = timed(1)
+    out["persistent_ms_per_batch"] = wall * 1e3
+    log(f"[bench] batch64 persistent launch: {wall * 1e3:.0f} ms per batch")
     t = time.perf_counter()
-    for _ in range(steps):
-        st = br.render()
-        rays += st["rays"]
-        samples += st["samples"]
-    wall = time.perf_counter() - t
-    log(f"[bench] batch64: {samples / steps / 1e6:.0f} M samples per batch, {wall / steps * 1e3:.0f} ms per batch")
-    return {"scenes": 64, "triangles": int(sum(s.n_tris for s in scs)), "samples_per_batch": samples // steps,
-            "ms_per_batch": wall / steps * 1e3, "rays_per_s": rays / wall, "samples_per_s": samples / wall,
-            "host_tree_build_s": build_s}
+    tuned, per_path_ms = br.autotune(passes=2)
+    out["autotune_s"] = time.perf_counter() - t
+    out["autotuned"] = tuned
+    log(f"[bench] batch64 autotuned in {out['autotune_s']:.1f} s: {tuned}")
+    rays, samples, wall = timed(steps)
+    out.update({"samples_per_batch": samples // steps, "ms_per_batch": wall / steps * 1e3, "rays_per_s": rays / wall,
+                "samples_per_s": samples / wall})
+    log(f"[bench] batch64 autotuned: {samples / steps / 1e6:.0f} M samples per batch, {wall / steps * 1e3:.0f} ms")
+    br.set_tuning()
+    return out
 
 
 def bpt_cornell(res=256, spp=64, cpu=True):

@@ -106,7 +106,13 @@ typedef struct rx_render_tuning {
                              min(spp, path_slots / pixels) samples of every pixel (>= 1) */
   uint32_t blocks_per_cu; /* resident 128-thread blocks per CU of the render kernels
                              (default and maximum: the traversal kernel's occupancy limit) */
-  uint32_t use_graph;     /* 1: record one pass into a hipGraph and replay it for every pass */
+  uint32_t use_graph;     /* rx_batch_render: 1 records one pass into a hipGraph and replays it for every pass;
+                             2 runs the whole batch (every pass and bounce) as one persistent cooperative launch
+                             with grid-wide barriers between the stages.  rx_render: ignored */
+  uint32_t stack_depth;   /* LDS traversal stack entries per lane of the extend kernel: 0 (default 10), 6, 8, 12, 16
+                             with block_size 0/128; deeper entries spill to a per-thread global stack */
+  uint32_t block_size;    /* threads per block of the extend kernel: 0 (default 128), 64 or 256 (with the default
+                             stack_depth); the grid keeps the default shape's thread count */
   uint32_t reserved;
 } rx_render_tuning;
 
@@ -266,6 +272,14 @@ int rx_batch_add_scene(rx_batch* batch, uint32_t n_verts, const float* verts, ui
                        const rx_camera_desc* camera, const rx_render_params* params, uint32_t* index);
 /* Renders every queued scene; films stay on the device. */
 int rx_batch_render(rx_batch* batch, rx_render_stats* stats);
+/* Times the first `passes` passes of rx_batch_render (0: all) under each
+ * candidate launch shape and keeps the fastest (time per camera path) as the
+ * context's rx_render_tuning: path slots (the wavefront size, which is also
+ * every queue's capacity) x blocks per CU with graph replay, then the extend
+ * kernel's stack depth and block size, then stream launches and the one
+ * persistent launch.  `best` and `best_ms` (ms per camera path) may be NULL.
+ * No counterpart in the reference. */
+int rx_batch_autotune(rx_batch* batch, uint32_t passes, rx_render_tuning* best, float* best_ms);
 /* One scene's film as rx_render returns it (any output may be NULL). */
 int rx_batch_get_image(rx_batch* batch, uint32_t index, float* rgb_mean, double* sum, double* sumsq);
 

@@ -47,6 +47,7 @@ SIGNATURES = {
     "rx_batch_destroy": (None, [P]),
     "rx_batch_add_scene": (C.c_int, [P, U32, P, U32, P, U32, P, U32, P, P, P]),
     "rx_batch_render": (C.c_int, [P, P]),
+    "rx_batch_autotune": (C.c_int, [P, U32, P, P]),
     "rx_batch_get_image": (C.c_int, [P, U32, P, P, P]),
     "rx_save_pfm": (C.c_int, [C.c_char_p, I32, I32, P]),
     "rx_save_pfm_snapshot": (C.c_int, [C.c_char_p, U32, U32, I32, I32, P]),

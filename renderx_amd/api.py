@@ -26,10 +26,18 @@ RENDER_STATS = np.dtype([("rays", "<u8"), ("extend_rays", "<u8"), ("shadow_rays"
                          ("ms_extend", "<f8"), ("ms_shade", "<f8"), ("ms_shadow", "<f8"),
                          ("extend_launches", "<u8"), ("guide_rows", "<u8"), ("guide_train_rows", "<u8")])
 RENDER_TIMING, RENDER_COUNT_WORK = 1, 2
-RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use_graph", "<u4"), ("reserved", "<u4")])
+RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use_graph", "<u4"), ("stack_depth", "<u4"),
+                          ("block_size", "<u4"), ("reserved", "<u4")])
 GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("batch_size", "<u4"), ("steps_per_pass", "<u4"), ("learning_rate", "<f4"),
                          ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4")])
 MODE_PT, MODE_NEE, MODE_GUIDED, MODE_BPT = 0, 1, 2, 3
+
+
+def _tuning(path_slots, blocks_per_cu, use_graph, stack_depth, block_size):
+    t = np.zeros(1, RENDER_TUNING)
+    t["path_slots"], t["blocks_per_cu"], t["use_graph"] = path_slots, blocks_per_cu, use_graph
+    t["stack_depth"], t["block_size"] = stack_depth, block_size
+    return t
 GUIDE_LR_DECAY = 1
 
 
@@ -152,11 +160,11 @@ class PathTracerHIP:
         p["seed"] = seed
         self.ctx.check(lib().rx_guide_init(self.ctx.h, ptr(p)), "rx_guide_init")
 
-    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0):
+    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0, stack_depth=0, block_size=0):
         """Launch shape of renderPixels (rx_render_tuning); images do not depend on it."""
-        t = np.zeros(1, RENDER_TUNING)
-        t["path_slots"], t["blocks_per_cu"], t["use_graph"] = path_slots, blocks_per_cu, use_graph
-        self.ctx.check(lib().rx_set_render_tuning(self.ctx.h, ptr(t)), "rx_set_render_tuning")
+        self.ctx.check(lib().rx_set_render_tuning(self.ctx.h, ptr(_tuning(path_slots, blocks_per_cu, use_graph,
+                                                                          stack_depth, block_size))),
+                       "rx_set_render_tuning")
 
     def get_tuning(self):
         t = np.zeros(1, RENDER_TUNING)
@@ -242,11 +250,23 @@ class BatchRenderer:
         self.shapes.append((int(cam["height"][0]), int(cam["width"][0])))
         return int(idx.value)
 
-    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0):
-        """Launch shape (rx_render_tuning of the context); images do not depend on it."""
+    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0, stack_depth=0, block_size=0):
+        """Launch shape (rx_render_tuning of the context); images do not depend on it.
+        use_graph: 0 stream launches, 1 one pass recorded in a hipGraph and
+        replayed, 2 the whole batch as one persistent cooperative launch."""
+        self.ctx.check(lib().rx_set_render_tuning(self.ctx.h, ptr(_tuning(path_slots, blocks_per_cu, use_graph,
+                                                                          stack_depth, block_size))),
+                       "rx_set_render_tuning")
+
+    def autotune(self, passes=2):
+        """rx_batch_autotune: times the first `passes` passes of the batch under
+        each candidate launch shape (path slots, blocks per CU, extend stack
+        depth and block size, graph replay or one persistent launch) and keeps
+        the fastest as the context's tuning.  Returns (tuning, ms)."""
         t = np.zeros(1, RENDER_TUNING)
-        t["path_slots"], t["blocks_per_cu"], t["use_graph"] = path_slots, blocks_per_cu, use_graph
-        self.ctx.check(lib().rx_set_render_tuning(self.ctx.h, ptr(t)), "rx_set_render_tuning")
+        ms = np.zeros(1, np.float32)
+        self.ctx.check(lib().rx_batch_autotune(self.h, passes, ptr(t), ptr(ms)), "rx_batch_autotune")
+        return {k: int(t[k][0]) for k in RENDER_TUNING.names if k != "reserved"}, float(ms[0])
 
     def render(self):
         st = np.zeros(1, RENDER_STATS)

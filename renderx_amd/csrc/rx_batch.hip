@@ -12,6 +12,7 @@
 // order, so the FP64 film adds every pixel's samples in the same order as a
 // single-scene render.  The schedule lives on the device and a device counter
 // selects the pass, so one pass is captured into a hipGraph and replayed.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +30,45 @@ struct BatchItem {
   uint32_t npix;
   SceneInfo info;
 };
+
+// The whole batch as one launch: the grid stays resident (cooperative
+// launch) and runs every pass's stages in order, generate, then per bounce
+// extend, shade and (NEE) shadow, then the film, with a grid-wide barrier
+// where the stream launches have a kernel boundary.  The stages are the
+// stream kernels' own bodies; one LDS buffer serves them in turn.
+template <int MODE>
+union PersistLds {
+  uint2 stack[kLdsDepth * kRBlock];
+  ShadeLds<MODE> shade;
+};
+constexpr int kPersistMinWaves = 4;  // resident waves per SIMD the registers must allow
+template <int MODE>
+__global__ void __launch_bounds__(kRBlock, kPersistMinWaves)
+    k_batch_persistent(DevScene S, PathBufs B, PassArgs P, BatchArgs BA, uint32_t n_passes, uint32_t Dmax) {
+  __shared__ PersistLds<MODE> sh;
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const GuideArgs GA{};
+  const DevCam C0{};
+  for (uint32_t k = 0; k < n_passes; k++) {  // every block runs the same barriers
+    generate_body<true, false>(S, C0, B, P, BA, sh.stack);
+    grid.sync();
+    for (uint32_t bb = 0; bb <= Dmax; bb++) {
+      if (bb == 0) extend_body<false, false, true, kLdsDepth>(S.tree, B, bb, BA, sh.stack);
+      else extend_body<true, false, true, kLdsDepth>(S.tree, B, bb, BA, sh.stack);
+      grid.sync();
+      shade_body<MODE, true, false>(S, B, P, bb, GA, BA, sh.shade);
+      grid.sync();
+      if (MODE == RX_MODE_NEE) {
+        shadow_body<true, false>(S, B, P, bb, GA, BA, sh.stack);
+        grid.sync();
+      }
+    }
+    accumulate_batch_body(B, BA);
+    grid.sync();
+    if (blockIdx.x == 0 && threadIdx.x == 0) *BA.pass_id += 1;
+    grid.sync();
+  }
+}
 
 }  // namespace rx
 
@@ -170,8 +210,10 @@ int rx_batch_add_scene(rx_batch* b, uint32_t n_verts, const float* verts, uint32
   return RX_OK;
 }
 
-int rx_batch_render(rx_batch* b, rx_render_stats* stats) {
-  if (!b) return RX_ERR_INVALID;
+namespace {
+// rx_batch_render; max_passes > 0 renders only the schedule's first passes
+// (the autotuner's timing runs).
+int batch_render(rx_batch* b, rx_render_stats* stats, uint32_t max_passes) {
   rx_ctx* c = b->c;
   if (b->items.empty()) return set_err(c, RX_ERR_STATE, "rx_batch_render: no scenes");
   (void)hipSetDevice(c->device);
@@ -313,13 +355,12 @@ int rx_batch_render(rx_batch* b, rx_render_stats* stats) {
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, B.stats, 8, B.overflow);
   hipLaunchKernelGGL(k_pass_set, dim3(1), dim3(64), 0, s, BA.pass_id, 0u);
   RXCK(hipGetLastError(), "launch fill");
+  const uint32_t sdep = c->tuning.stack_depth, sblk = c->tuning.block_size;
   auto launch_pass = [&]() {
     hipLaunchKernelGGL((k_generate<true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, C0, B, P, BA);
     for (uint32_t bb = 0; bb <= Dmax; bb++) {
-      if (bb == 0)
-        hipLaunchKernelGGL((k_extend<false, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, bb, BA);
-      else
-        hipLaunchKernelGGL((k_extend<true, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, bb, BA);
+      if (bb == 0) launch_extend<false, false, true>(sdep, sblk, blocks, s, S.tree, B, bb, BA);
+      else launch_extend<true, false, true>(sdep, sblk, blocks, s, S.tree, B, bb, BA);
       if (b->mode == RX_MODE_NEE) {
         hipLaunchKernelGGL((k_shade<1, true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
         hipLaunchKernelGGL((k_shadow<true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
@@ -330,21 +371,32 @@ int rx_batch_render(rx_batch* b, rx_render_stats* stats) {
     hipLaunchKernelGGL(k_accumulate_batch, dim3(blocks), dim3(kRBlock), 0, s, B, BA);
     hipLaunchKernelGGL(k_pass_next, dim3(1), dim3(64), 0, s, BA.pass_id);
   };
+  const size_t n_passes = max_passes ? std::min<size_t>(max_passes, passes.size()) : passes.size();
   RXCK(hipEventRecord(c->ev0, s), "event");
-  if (c->tuning.use_graph) {  // record one pass, replay it for every pass
+  if (c->tuning.use_graph == 2) {  // one persistent cooperative launch for every pass
+    void* fn = b->mode == RX_MODE_NEE ? (void*)k_batch_persistent<RX_MODE_NEE> : (void*)k_batch_persistent<RX_MODE_PT>;
+    int nb = 0;
+    RXCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kRBlock, 0), "persistent occupancy");
+    if (nb < 1) return set_err(c, RX_ERR_INVALID, "rx_batch_render: the persistent kernel cannot be resident");
+    // the grid must be resident at once; it also must not exceed the spill buffer's threads
+    const int pgrid = std::min(blocks, c->n_cu * std::min(nb, per_cu));
+    uint32_t np = (uint32_t)n_passes, dm = Dmax;
+    void* args[] = {(void*)&S, (void*)&B, (void*)&P, (void*)&BA, (void*)&np, (void*)&dm};
+    RXCK(hipLaunchCooperativeKernel(fn, dim3(pgrid), dim3(kRBlock), args, 0, s), "persistent launch");
+  } else if (c->tuning.use_graph) {  // record one pass, replay it for every pass
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     RXCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
     launch_pass();
     hipError_t e = hipStreamEndCapture(s, &graph);
     if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    for (size_t k = 0; e == hipSuccess && k < passes.size(); k++) e = hipGraphLaunch(exec, s);
+    for (size_t k = 0; e == hipSuccess && k < n_passes; k++) e = hipGraphLaunch(exec, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     RXCK(e, "pass graph");
   } else {
-    for (size_t k = 0; k < passes.size(); k++) launch_pass();
+    for (size_t k = 0; k < n_passes; k++) launch_pass();
     RXCK(hipGetLastError(), "launch pass");
   }
   RXCK(hipEventRecord(c->ev1, s), "event");
@@ -361,14 +413,90 @@ int rx_batch_render(rx_batch* b, rx_render_stats* stats) {
     stats->extend_rays = st[1];
     stats->shadow_rays = st[2];
     stats->segments = st[3];
-    uint64_t samples = 0;
-    for (auto& it : b->items) samples += (uint64_t)it.npix * it.prm.spp;
+    uint64_t samples = 0;  // camera paths of the passes rendered (all passes: every scene's pixels x spp)
+    for (size_t k = 0; k < n_passes; k++) samples += passes[k].z;
     stats->samples = samples;
     stats->ms_total = ms;
-    stats->extend_launches = (uint64_t)passes.size() * (Dmax + 1);
+    stats->extend_launches = (uint64_t)n_passes * (Dmax + 1);
   }
 #undef RXCK
   if (ovf) return set_err(c, RX_ERR_OVERFLOW, "rx_batch_render: traversal stack overflow");
+  return RX_OK;
+}
+}  // namespace
+
+int rx_batch_render(rx_batch* b, rx_render_stats* stats) {
+  if (!b) return RX_ERR_INVALID;
+  return batch_render(b, stats, 0);
+}
+
+int rx_batch_autotune(rx_batch* b, uint32_t passes, rx_render_tuning* best, float* best_ms) {
+  if (!b) return RX_ERR_INVALID;
+  rx_ctx* c = b->c;
+  if (b->items.empty()) return set_err(c, RX_ERR_STATE, "rx_batch_autotune: no scenes");
+  const rx_render_tuning keep = c->tuning;
+  rx_render_tuning win = keep;
+  float win_ms = 0.0f;
+  bool have = false;
+  auto trial = [&](const rx_render_tuning& t) -> int {
+    c->tuning = t;
+    rx_render_stats st;
+    int r = batch_render(b, &st, passes);
+    if (r == RX_OK) r = batch_render(b, &st, passes);  // the second run is timed (first-use costs excluded)
+    if (r) {
+      c->tuning = keep;
+      return r;
+    }
+    // per pass: the candidates differ in path slots, so in their pass counts
+    const float per = (float)(st.ms_total / std::max<double>(1.0, (double)st.samples));
+    if (!have || per < win_ms) {
+      win = t;
+      win_ms = per;
+      have = true;
+    }
+    return RX_OK;
+  };
+  if (!c->render_blocks_per_cu) {  // occupancy limit, set by a first render
+    rx_render_tuning t = keep;
+    c->tuning = t;
+    rx_render_stats st;
+    if (int r = batch_render(b, &st, 1)) {
+      c->tuning = keep;
+      return r;
+    }
+  }
+  const int occ = c->render_blocks_per_cu;
+  // coordinate search: wavefront size (path slots, also the queue capacity) x
+  // blocks per CU with graph replay; then the extend kernel's (stack depth,
+  // block size); then the launch form (stream launches, one persistent launch)
+  const uint32_t slot_opts[] = {1u << 23, 1u << 24, 1u << 25, 1u << 26};
+  const int blk_opts[] = {occ, std::max(2, (3 * occ) / 4)};
+  for (uint32_t sl : slot_opts)
+    for (int bk : blk_opts) {
+      rx_render_tuning t{};
+      t.path_slots = sl;
+      t.blocks_per_cu = (uint32_t)bk;
+      t.use_graph = 1;
+      if (int r = trial(t)) return r;
+    }
+  rx_render_tuning base = win;
+  for (const ExtendShape& e : kExtendShapes) {
+    if (e.depth == kLdsDepth && e.block == kRBlock) continue;
+    rx_render_tuning t = base;
+    t.stack_depth = (uint32_t)e.depth;
+    t.block_size = (uint32_t)e.block;
+    if (int r = trial(t)) return r;
+  }
+  base = win;
+  for (uint32_t g : {0u, 2u}) {
+    rx_render_tuning t = base;
+    t.use_graph = g;
+    if (g == 2) t.stack_depth = t.block_size = 0;  // the persistent kernel has the default extend shape
+    if (int r = trial(t)) return r;
+  }
+  c->tuning = win;
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;  // per sample
   return RX_OK;
 }
 

@@ -85,7 +85,7 @@ hipError_t TreeBufs::upload(const DeviceTreeHost& h, uint32_t depth, hipStream_t
 }
 
 int ensure_spill(rx_ctx* c, int grid_blocks, int block, int depth_needed, int* spill_depth) {
-  int sd = std::max(1, depth_needed + 2 - kLdsDepth);
+  int sd = std::max(1, depth_needed + 2 - kMinLdsDepth);
   size_t bytes = (size_t)grid_blocks * block * sd * sizeof(uint2);
   if (int r = hip_err(c, c->spill.ensure(bytes), "spill alloc")) return r;
   if (int r = hip_err(c, c->overflow.ensure(4 * (1 + kWorkWords)), "overflow alloc")) return r;

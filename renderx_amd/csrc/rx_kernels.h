@@ -12,6 +12,9 @@ constexpr int kTraceBlock = 128;   // threads per traversal block (2 waves)
 #define RX_LDS_DEPTH 10
 #endif
 constexpr int kLdsDepth = RX_LDS_DEPTH;  // LDS stack entries per lane
+// the smallest LDS stack depth of any extend-kernel launch shape: the
+// per-thread spill buffer is sized for it
+constexpr int kMinLdsDepth = kLdsDepth < 6 ? kLdsDepth : 6;
 
 struct IsectArgs {
   uint32_t n;

@@ -351,8 +351,8 @@ __device__ __forceinline__ float4 f4(Vec3 v, float w) { return make_float4(v.x, 
 // camera cosine and the vignetting division of src/PathTracer.cpp:55,67
 // (Camera::eliminateVignetting, src/Camera.cpp:78-96) start the throughput.
 template <bool BATCH, bool VOL>
-__global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C0, PathBufs B, PassArgs P, BatchArgs BA) {
-  __shared__ uint2 stack[kLdsDepth * kRBlock];
+__device__ __forceinline__ void generate_body(DevScene S, DevCam C0, PathBufs B, PassArgs P, BatchArgs BA,
+                                              uint2* stack) {
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t rays = 0;
   uint32_t n_paths = P.n_paths;
@@ -407,19 +407,26 @@ __global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C0, Pat
   wave_add(&B.stats[0], rays);
 }
 
+template <bool BATCH, bool VOL>
+__global__ void __launch_bounds__(kRBlock) k_generate(DevScene S, DevCam C0, PathBufs B, PassArgs P, BatchArgs BA) {
+  __shared__ uint2 stack[kLdsDepth * kRBlock];
+  generate_body<BATCH, VOL>(S, C0, B, P, BA, stack);
+}
+
 // ------------------------------------------------------------------ extend
 // Nearest hit of every queued path: the waves take chunks of the queue from
 // the bounce's work counter (trace_range); every ray's result is independent
 // of the lane that ran it.
-template <bool LEG, bool STATS, bool BATCH>
-__global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend(DevTree T, PathBufs B, uint32_t b, BatchArgs BA) {
-  __shared__ uint2 stack[kLdsDepth * kRBlock];
+// DEPTH: LDS stack entries per lane; BLOCK: threads per block (the launch
+// shapes the autotuner chooses between, see launch_extend).
+template <bool LEG, bool STATS, bool BATCH, int DEPTH>
+__device__ __forceinline__ void extend_body(DevTree T, PathBufs B, uint32_t b, BatchArgs BA, uint2* stack) {
   const uint32_t n = B.cnt[b];
   const uint32_t* q = (b & 1) ? B.q1 : B.q0;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t boxes = 0, tris = 0;
   bool ovf = false;
-  trace_range<false, LEG, STATS, kLdsDepth>(
+  trace_range<false, LEG, STATS, DEPTH>(
       T, n, B.cnt + kWorkBase + b * kWorkWords, stack + threadIdx.x, B.spill + (size_t)gtid * B.spill_depth, B.spill_depth, &ovf,
       [&](uint32_t i, Vec3& o, Vec3& d, int32_t& ins, int32_t& con, TreeRoot& root) {
         const uint32_t p = q[i];
@@ -455,6 +462,50 @@ __global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend(DevTree T, 
     atomicAdd(&B.stats[0], (unsigned long long)n);
     atomicAdd(&B.stats[1], (unsigned long long)n);
   }
+}
+
+template <bool LEG, bool STATS, bool BATCH, int DEPTH = kLdsDepth, int BLOCK = kRBlock>
+__global__ void __launch_bounds__(BLOCK, kExtendMinWaves) k_extend(DevTree T, PathBufs B, uint32_t b, BatchArgs BA) {
+  __shared__ uint2 stack[DEPTH * BLOCK];
+  extend_body<LEG, STATS, BATCH, DEPTH>(T, B, b, BA, stack);
+}
+
+// Launch shapes of the extend kernel: (LDS stack depth, block size) pairs
+// with an instantiation.  Every shape gives the same hits; the grid keeps
+// the thread count of the default shape (blocks * kRBlock threads), which
+// the spill buffer is sized for.
+struct ExtendShape {
+  int depth, block;
+};
+constexpr ExtendShape kExtendShapes[] = {{kLdsDepth, kRBlock}, {6, 128}, {8, 128}, {12, 128}, {16, 128},
+                                         {kLdsDepth, 64}, {kLdsDepth, 256}};
+inline bool extend_shape_ok(uint32_t depth, uint32_t block) {
+  for (const ExtendShape& e : kExtendShapes)
+    if ((uint32_t)e.depth == (depth ? depth : (uint32_t)kLdsDepth) && (uint32_t)e.block == (block ? block : (uint32_t)kRBlock))
+      return true;
+  return false;
+}
+template <bool LEG, bool STATS, bool BATCH>
+inline void launch_extend(uint32_t depth, uint32_t block, int blocks, hipStream_t s, const DevTree& T,
+                          const PathBufs& B, uint32_t b, const BatchArgs& BA) {
+  const int d = depth ? (int)depth : kLdsDepth, k = block ? (int)block : kRBlock;
+  const int threads = blocks * kRBlock;
+#define RX_EXTEND_SHAPE(D, K)                                                                                  \
+  if (d == D && k == K) {                                                                                      \
+    hipLaunchKernelGGL((k_extend<LEG, STATS, BATCH, D, K>), dim3(std::max(1, threads / K)), dim3(K), 0, s, T, B, \
+                       b, BA);                                                                                 \
+    return;                                                                                                    \
+  }
+  if constexpr (!STATS) {  // the counting renders are untimed: default shape only
+    RX_EXTEND_SHAPE(6, 128)
+    RX_EXTEND_SHAPE(8, 128)
+    RX_EXTEND_SHAPE(12, 128)
+    RX_EXTEND_SHAPE(16, 128)
+    RX_EXTEND_SHAPE(kLdsDepth, 64)
+    RX_EXTEND_SHAPE(kLdsDepth, 256)
+  }
+#undef RX_EXTEND_SHAPE
+  hipLaunchKernelGGL((k_extend<LEG, STATS, BATCH>), dim3(blocks), dim3(kRBlock), 0, s, T, B, b, BA);
 }
 
 // ------------------------------------------------------------------ guide queries
@@ -797,11 +848,16 @@ __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3
 #ifndef RX_SHADE_MIN_WAVES
 #define RX_SHADE_MIN_WAVES 6
 #endif
+template <int MODE>
+struct ShadeLds {
+  uint2 stack[kLdsDepth * kRBlock];
+  uint32_t cont_list[kStageCap], nee_list[MODE >= 1 ? kStageCap : 1], counts[4];
+};
 template <int MODE, bool BATCH, bool VOL>
-__global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WAVES : 1) k_shade(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
-                                                   BatchArgs BA) {
-  __shared__ uint2 stack[kLdsDepth * kRBlock];
-  __shared__ uint32_t cont_list[kStageCap], nee_list[MODE >= 1 ? kStageCap : 1], counts[4];
+__device__ __forceinline__ void shade_body(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA, BatchArgs BA,
+                                           ShadeLds<MODE>& sh) {
+  uint2* stack = sh.stack;
+  uint32_t *cont_list = sh.cont_list, *nee_list = sh.nee_list, *counts = sh.counts;
   const StagedQueue qc{cont_list, counts}, qn{nee_list, counts + 2};
   const uint32_t n = B.cnt[b];
   const uint32_t* qin = (b & 1) ? B.q1 : B.q0;
@@ -1135,6 +1191,13 @@ __global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WA
   wave_add(&B.stats[3], segs);
 }
 
+template <int MODE, bool BATCH, bool VOL>
+__global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WAVES : 1)
+    k_shade(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA, BatchArgs BA) {
+  __shared__ ShadeLds<MODE> sh;
+  shade_body<MODE, BATCH, VOL>(S0, B, P, b, GA, BA, sh);
+}
+
 // ------------------------------------------------------------------ shadow
 // Visibility as MCRenderer::testVisibility (src/MCRenderer.cpp:108-120):
 // visible when the first legal hit is not nearer than the light point
@@ -1144,9 +1207,8 @@ __global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WA
 // after c crossings from the vertex of bounce b counts only when
 // b + 1 + c <= max_depth, the paths the PathTracer itself can produce.
 template <bool BATCH, bool VOL>
-__global__ void __launch_bounds__(kRBlock, kShadowMinWaves) k_shadow(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA,
-                                                    BatchArgs BA) {
-  __shared__ uint2 stack[kLdsDepth * kRBlock];
+__device__ __forceinline__ void shadow_body(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA, BatchArgs BA,
+                                            uint2* stack) {
   const uint32_t n = B.cnt[kShadowBase + b];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   uint2* lds = stack + threadIdx.x;
@@ -1257,6 +1319,13 @@ __global__ void __launch_bounds__(kRBlock, kShadowMinWaves) k_shadow(DevScene S0
   if (ovf) *B.overflow = 1;
   wave_add(&B.stats[0], rays);
   wave_add(&B.stats[2], srays);
+}
+
+template <bool BATCH, bool VOL>
+__global__ void __launch_bounds__(kRBlock, kShadowMinWaves)
+    k_shadow(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA, BatchArgs BA) {
+  __shared__ uint2 stack[kLdsDepth * kRBlock];
+  shadow_body<BATCH, VOL>(S0, B, P, b, GA, BA, stack);
 }
 
 // ------------------------------------------------------------------ training samples
@@ -1451,7 +1520,7 @@ static __global__ void __launch_bounds__(kAccPix) k_accumulate(PathBufs B, PassA
 // Batch film: a pass holds at most one sample of any pixel and passes run in
 // sample order, so each slot adds to its own pixel and the sums are formed in
 // the same order as a single-scene render of that scene.
-static __global__ void k_accumulate_batch(PathBufs B, BatchArgs BA) {
+__device__ __forceinline__ void accumulate_batch_body(PathBufs B, BatchArgs BA) {
   const uint32_t n = BA.passes[*BA.pass_id].z;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const size_t q = (size_t)BA.info[BA.scn[i]].film_base + BA.pix[i];
@@ -1463,6 +1532,7 @@ static __global__ void k_accumulate_batch(PathBufs B, BatchArgs BA) {
     }
   }
 }
+static __global__ void k_accumulate_batch(PathBufs B, BatchArgs BA) { accumulate_batch_body(B, BA); }
 static __global__ void k_pass_set(uint32_t* pass_id, uint32_t v) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *pass_id = v;
 }

@@ -337,12 +337,13 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     else hipLaunchKernelGGL((k_generate<false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, BA0);
     for (uint32_t b = 0; b <= prm->max_depth; b++) {
       hipEvent_t e0 = timing ? mark() : nullptr;
+      const uint32_t sdep = c->tuning.stack_depth, sblk = c->tuning.block_size;
       if (b == 0) {
-        if (count) hipLaunchKernelGGL((k_extend<false, true, false>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, b, BA0);
-        else hipLaunchKernelGGL((k_extend<false, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, b, BA0);
+        if (count) launch_extend<false, true, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
+        else launch_extend<false, false, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
       } else {
-        if (count) hipLaunchKernelGGL((k_extend<true, true, false>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, b, BA0);
-        else hipLaunchKernelGGL((k_extend<true, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, b, BA0);
+        if (count) launch_extend<true, true, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
+        else launch_extend<true, false, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
       }
       launches++;
       hipEvent_t e1 = timing ? mark() : nullptr;
@@ -450,7 +451,10 @@ int rx_set_render_tuning(rx_ctx* c, const rx_render_tuning* t) {
   if (t->path_slots && (t->path_slots < 4096 || t->path_slots > (1u << 27)))
     return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: path_slots must be 0 or in [4096, 2^27]");
   if (t->blocks_per_cu > 64) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: blocks_per_cu > 64");
-  if (t->use_graph > 1) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: use_graph is 0 or 1");
+  if (t->use_graph > 2) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: use_graph is 0, 1 or 2");
+  if (!extend_shape_ok(t->stack_depth, t->block_size))
+    return set_err(c, RX_ERR_INVALID,
+                   "rx_set_render_tuning: (stack_depth, block_size) must be (0|10, 0|64|128|256) or (6|8|12|16, 0|128)");
   c->tuning = *t;
   return RX_OK;
 }
@@ -479,6 +483,22 @@ int rx_autotune_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_par
   rx_render_tuning win = keep;
   float win_ms = 0.0f;
   bool have = false;
+  auto trial = [&](const rx_render_tuning& t) -> int {
+    c->tuning = t;
+    rx_render_stats st;
+    if (int r = rx_render(c, cam, &p, nullptr, nullptr, nullptr, &st)) {
+      c->tuning = keep;
+      return r;
+    }
+    if (!have || st.ms_total < win_ms) {
+      win = t;
+      win_ms = (float)st.ms_total;
+      have = true;
+    }
+    return RX_OK;
+  };
+  // coordinate search: path slots x blocks per CU at the default extend
+  // shape, then the extend kernel's (stack depth, block size) at the winner
   const uint32_t slot_opts[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
   const int blk_opts[] = {occ, std::max(2, (3 * occ) / 4)};
   for (uint32_t sl : slot_opts) {
@@ -486,18 +506,16 @@ int rx_autotune_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_par
       rx_render_tuning t{};
       t.path_slots = sl;
       t.blocks_per_cu = (uint32_t)bk;
-      c->tuning = t;
-      rx_render_stats st;
-      if (int r = rx_render(c, cam, &p, nullptr, nullptr, nullptr, &st)) {
-        c->tuning = keep;
-        return r;
-      }
-      if (!have || st.ms_total < win_ms) {
-        win = t;
-        win_ms = (float)st.ms_total;
-        have = true;
-      }
+      if (int r = trial(t)) return r;
     }
+  }
+  const rx_render_tuning base = win;
+  for (const ExtendShape& e : kExtendShapes) {
+    if (e.depth == kLdsDepth && e.block == kRBlock) continue;  // timed above
+    rx_render_tuning t = base;
+    t.stack_depth = (uint32_t)e.depth;
+    t.block_size = (uint32_t)e.block;
+    if (int r = trial(t)) return r;
   }
   c->tuning = win;
   if (best) *best = win;

@@ -33,8 +33,13 @@ def _scene_set():
     return out
 
 
+# (path slots, extend stack depth, extend block size); 4096 slots split the
+# larger scenes over several passes
+SHAPES = [(0, 0, 0), (4096, 0, 0), (0, 8, 128), (4096, 16, 128), (0, 10, 256), (0, 10, 64)]
+
+
 @pytest.mark.parametrize("mode", [MODE_PT, MODE_NEE])
-@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("graph", [0, 1, 2])
 def test_batch_images_equal_single_renders(rx, gpu_ctx, mode, graph):
     print(f"[batch] mode={mode} graph={graph}", flush=True)
     scs = _scene_set()
@@ -46,8 +51,10 @@ def test_batch_images_equal_single_renders(rx, gpu_ctx, mode, graph):
     for k, sc in enumerate(scs):
         assert br.add(sc, mode=mode, seed=100 + k) == k
     try:
-        for slots in (0, 4096):  # 4096 slots split the larger scenes over several passes
-            pt.set_tuning(path_slots=slots, use_graph=graph)
+        for slots, depth, block in SHAPES:
+            if graph == 2 and (depth or block):
+                continue  # the persistent launch has the default extend shape
+            pt.set_tuning(path_slots=slots, use_graph=graph, stack_depth=depth, block_size=block)
             st = br.render()
             assert st["samples"] == sum(sc.camera["width"] * sc.camera["height"] * sc.spp for sc in scs)
             for k in range(len(scs)):
@@ -56,9 +63,31 @@ def test_batch_images_equal_single_renders(rx, gpu_ctx, mode, graph):
                 assert np.array_equal(img["sum"], a["sum"]), (k, slots)
                 assert np.array_equal(img["sumsq"], a["sumsq"]), (k, slots)
                 assert np.array_equal(img["mean"], a["mean"]), (k, slots)
-            print(f"  slots={slots}: {len(scs)} scenes identical, {st['rays']} rays in {st['ms_total']:.2f} ms")
+            print(f"  slots={slots} depth={depth} block={block}: {len(scs)} scenes identical, {st['rays']} rays "
+                  f"in {st['ms_total']:.2f} ms")
     finally:
         pt.set_tuning()
+
+
+def test_batch_autotune(rx, gpu_ctx):
+    """rx_batch_autotune times the candidate shapes on the first passes, keeps
+    one that the context accepts, and the films stay bit-identical to the
+    single renders under it."""
+    print("[batch] autotune", flush=True)
+    scs = _scene_set()
+    single = [rx.PathTracerHIP(sc, ctx=gpu_ctx).renderPixels(mode=MODE_NEE, seed=100 + k) for k, sc in enumerate(scs)]
+    br = rx.BatchRenderer(ctx=gpu_ctx)
+    for k, sc in enumerate(scs):
+        br.add(sc, mode=MODE_NEE, seed=100 + k)
+    try:
+        t, ms = br.autotune(passes=2)
+        print(f"  chose {t}, {ms * 1e6:.2f} ns per camera path")
+        assert ms > 0 and t["use_graph"] in (0, 1, 2)
+        br.render()
+        for k in range(len(scs)):
+            assert np.array_equal(br.image(k)["sum"], single[k]["sum"]), k
+    finally:
+        br.set_tuning()
 
 
 def test_batch_rejects_mixed_modes_and_media(rx, gpu_ctx):

@@ -183,18 +183,24 @@ def test_absorbing_slab_closed_form(rx, gpu_ctx, mode):
 
 
 def test_launch_tuning_gives_identical_images(rx, gpu_ctx):
-    """rx_render_tuning (samples per pass, resident blocks) and the autotuner
-    change only the time: the film is bit-identical."""
+    """rx_render_tuning (samples per pass, resident blocks, the extend
+    kernel's stack depth and block size) and the autotuner change only the
+    time: the film is bit-identical."""
     print("[tuning]", flush=True)
     sc = scenes.media_scene(40, 40, freq=16, glass_freq=6)
     pt = rx.PathTracerHIP(sc, ctx=gpu_ctx)
     pt.set_tuning()
     a = pt.renderPixels(spp=48, mode=MODE_NEE, seed=3)
     try:
-        for slots, blk in [(4096, 0), (40 * 40 * 5, 2), (1 << 20, 3)]:
-            pt.set_tuning(path_slots=slots, blocks_per_cu=blk)
+        for slots, blk, depth, bsz in [(4096, 0, 0, 0), (40 * 40 * 5, 2, 0, 0), (1 << 20, 3, 0, 0), (0, 0, 6, 128),
+                                       (0, 0, 8, 0), (4096, 0, 12, 128), (0, 0, 16, 0), (0, 0, 10, 64),
+                                       (40 * 40 * 5, 0, 0, 256)]:
+            pt.set_tuning(path_slots=slots, blocks_per_cu=blk, stack_depth=depth, block_size=bsz)
             b = pt.renderPixels(spp=48, mode=MODE_NEE, seed=3)
-            assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sumsq"], b["sumsq"]), (slots, blk)
+            assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sumsq"], b["sumsq"]), (slots, blk, depth, bsz)
+        for depth, bsz in [(7, 0), (16, 256), (0, 32)]:  # shapes without an instantiation
+            with pytest.raises(rx.RxError):
+                pt.set_tuning(stack_depth=depth, block_size=bsz)
         best, ms = pt.autotune(spp=16, mode=MODE_NEE)
         print(f"  autotuned {best} in {ms:.2f} ms")
         assert best["path_slots"] > 0 and pt.get_tuning() == best
