@@ -138,6 +138,52 @@ def equal_time_relmse(res=512, ref_spp=65536, budget_s=5.0, chunk=64):
     return out
 
 
+def equal_time_glossy(res=512, ref_spp=65536, budget_s=3.0, chunk=64):
+    """The equal-time comparison on the glossy config: the 1,003,526-triangle
+    Phong-glossy displaced icosphere scene at 512x512, max depth 10.  The
+    converged image is NEE-only at 65,536 spp accumulated in FP64 (unbiased
+    for the same mean as plain path tracing, and with far less noise than a
+    plain render of that length).  Plain, NEE-only and guided rendering each
+    get the same wall-clock budget, training included (guided: 8 warm-up
+    passes of 2^18 samples, batch 16384 x 16 Adam steps, lr 1e-2)."""
+    from renderx_amd import api, scenes, film
+
+    sc = scenes.icosphere_scene(res, res, max_depth=10)
+    ctx = api.Context(0)
+    t = time.perf_counter()
+    ref_pt = api.PathTracerHIP(sc, ctx=ctx)
+    acc = None
+    step = 8192
+    for k in range(0, ref_spp, step):
+        r = ref_pt.renderPixels(spp=step, mode=api.MODE_NEE, seed=99, sample_offset=k)
+        acc = r["sum"] if acc is None else acc + r["sum"]
+        log(f"[bench] glossy relMSE reference: {k + step} of {ref_spp} spp ({time.perf_counter() - t:.0f} s)")
+    ref = (acc / ref_spp).astype(np.float32)
+    out = {"scene": f"Phong-glossy icosphere {sc.n_tris} tris, {res}x{res} px, max depth 10",
+           "reference": f"NEE-only, {ref_spp} spp, FP64 accumulation ({time.perf_counter() - t:.0f} s)",
+           "budget_s": budget_s}
+    for name, mode, guide in [("plain", api.MODE_PT, None), ("nee", api.MODE_NEE, None),
+                              ("guided", api.MODE_GUIDED,
+                               dict(train_passes=8, batch_size=16384, steps_per_pass=16, learning_rate=1e-2))]:
+        pt = api.PathTracerHIP(sc, ctx=ctx)
+        if guide:
+            pt.guide_init(seed=7, **guide)
+        acc, n = None, 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s:
+            r = pt.renderPixels(spp=chunk, mode=mode, seed=5, sample_offset=n)
+            acc = r["sum"] if acc is None else acc + r["sum"]
+            n += chunk
+        el = time.perf_counter() - t0
+        raw = film.relmse((acc / n).astype(np.float32), ref)
+        out[name] = raw * el / budget_s
+        out[name + "_spp"] = n
+        log(f"[bench] glossy {name}: {n} spp in {el:.2f} s, relMSE {raw:.7f} ({out[name]:.7f} at {budget_s:.0f} s)")
+    out["plain_over_guided"] = out["plain"] / out["guided"]
+    out["nee_over_guided"] = out["nee"] / out["guided"]
+    return out
+
+
 # FP32-accumulated BF16 MFMA work per network row (96 -> 64 -> 64 -> 64 -> 41):
 # the forward pass, and forward + input gradients of layers 2-4 + weight
 # gradients of all four layers in training
@@ -362,6 +408,7 @@ def main():
         log(f"[bench] calibration: rx::k_copy16 over {args.calib_copy_mib} MiB done")
     if not args.no_relmse:
         out["equal_time_relmse"] = equal_time_relmse()
+        out["equal_time_relmse_glossy"] = equal_time_glossy()
     if not args.no_batch:
         out["batch64"] = batch64()
     if not args.no_bpt:

@@ -185,6 +185,34 @@ __device__ __forceinline__ void stage_weights(const GuideNet& N, uint16_t* wts) 
   }
 }
 
+// Inference order of the encoding (k_guide_fwd): at k-step ks the lanes of
+// group lg (lane / 16) of a 16-row tile supply elements 2 ks and 2 ks + 1 of
+// their group's chain of 12 pairs: groups 0, 1, 2 the 12 position octaves of
+// axis lg, group 3 the 4 direction octaves of each axis in turn.  Along a
+// chain each octave follows from the previous one by the double-angle
+// formulas, so a lane evaluates sincospi once per axis; the first layer's
+// weight columns are staged in this order.
+__device__ __forceinline__ int enc_chain_pair(int lg, int m) {
+  return lg < 3 ? 3 * m + lg : 36 + 3 * (m & 3) + (m >> 2);
+}
+// stage_weights with the first layer's input columns in the chain order:
+// staged column 16 ks + 4 lg + 2 u + v holds feature 2 enc_chain_pair(lg, 2 ks + u) + v.
+__device__ __forceinline__ void stage_weights_chain(const GuideNet& N, uint16_t* wts) {
+  for (int e = threadIdx.x; e < kGHid * kGIn; e += blockDim.x) {
+    const int r = e / kGIn, c = e % kGIn;
+    const int ks = c >> 4, lg = (c >> 2) & 3, u = (c >> 1) & 1, v = c & 1;
+    wts[kWOff[0] + r * (kGIn + kWPad) + c] = N.Wt[0][(size_t)r * kGIn + 2 * enc_chain_pair(lg, 2 * ks + u) + v];
+  }
+  for (int L = 1; L < 4; L++) {
+    const int K = kLayerIn[L], NO = kLayerOut[L];
+    for (int e = threadIdx.x; e < NO * K / 4; e += blockDim.x) {  // 4 bf16 per step
+      const int r = (4 * e) / K, col = (4 * e) % K;
+      *reinterpret_cast<uint2*>(&wts[kWOff[L] + r * (K + kWPad) + col]) =
+          *reinterpret_cast<const uint2*>(N.Wt[L] + (size_t)r * K + col);
+    }
+  }
+}
+
 // Launchers (rx_guide.hip).  `rows` points at a device row count; the
 // kernels grid-stride over 16-row tiles so any count up to `cap` works.
 // Encode + forward of `rows` queries (pos[r * stride], dir[r * stride]):

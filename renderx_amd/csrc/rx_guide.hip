@@ -71,18 +71,36 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
   const uint32_t M = min(*rows, cap);
   const uint32_t nTiles = (M + 15) / 16;
   if (blockIdx.x * kFwdWaves >= nTiles) return;  // block-uniform
-  stage_weights(N, wts);
+  stage_weights_chain(N, wts);
   __syncthreads();
   for (uint32_t tile = blockIdx.x * kFwdWaves + wave; tile < nTiles; tile += gridDim.x * kFwdWaves) {
     const uint32_t r0 = tile * 16;
     const uint32_t gr = min(r0 + (uint32_t)lr, M - 1);
     const float4 p4 = pos[(size_t)gr * stride], d4 = dir[(size_t)gr * stride];
-    const Vec3 x = v3(p4.x, p4.y, p4.z), dd = v3(d4.x, d4.y, d4.z);
+    // the lane's chain (enc_chain_pair): a position coordinate in [0, 1]
+    // through 12 octaves, or the direction's three coordinates through 4 each
+    const float pc = lg == 0 ? p4.x : (lg == 1 ? p4.y : p4.z);
+    const float lo = N.lo[lg < 3 ? lg : 0], hi = N.hi[lg < 3 ? lg : 0];
+    const float px = fminf(1.0f, fmaxf(0.0f, (pc - lo) / (hi - lo)));
+    float sn, cs;
     v4s in[6];
 #pragma unroll
-    for (int ks = 0; ks < 6; ks++) {  // features 16 ks + 4 lg .. +3 = pairs 8 ks + 2 lg, +1
-      const uint32_t p0 = guide_pair(N, x, dd, 8 * ks + 2 * lg), p1 = guide_pair(N, x, dd, 8 * ks + 2 * lg + 1);
-      in[ks] = v4s{(short)(p0 & 0xffffu), (short)(p0 >> 16), (short)(p1 & 0xffffu), (short)(p1 >> 16)};
+    for (int ks = 0; ks < 6; ks++) {  // staged columns 16 ks + 4 lg .. +3: chain elements 2 ks, 2 ks + 1
+      uint32_t pr[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int m = 2 * ks + u;
+        if (m == 0 || (lg == 3 && (m & 3) == 0)) {  // start of an axis: sincospi of the coordinate
+          const float f = lg < 3 ? px : (m == 0 ? d4.x : (m == 4 ? d4.y : d4.z));
+          sincospif(f, &sn, &cs);
+        } else {  // the next octave: sin 2t = 2 sin t cos t, cos 2t = 1 - 2 sin^2 t
+          const float s2 = 2.0f * sn * cs, c2 = fmaf(-2.0f * sn, sn, 1.0f);
+          sn = s2;
+          cs = c2;
+        }
+        pr[u] = (uint32_t)f2bf(sn) | ((uint32_t)f2bf(cs) << 16);
+      }
+      in[ks] = v4s{(short)(pr[0] & 0xffffu), (short)(pr[0] >> 16), (short)(pr[1] & 0xffffu), (short)(pr[1] >> 16)};
     }
     v4f acc[4];
     v4s h[4];

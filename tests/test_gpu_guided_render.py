@@ -2,8 +2,9 @@
 the same mean as the plain path tracer (unbiasedness of the one-sample MIS
 estimator); at equal render time (training included) it beats both the plain
 GPU path tracer (by at least 2x) and NEE without guiding in relMSE against a
-converged image on the media + glass scene, and the trained network beats the
-same estimator with the untrained network."""
+converged image on the media + glass scene, beats the plain path tracer by at
+least 2x on the glossy config, and the trained network beats the same
+estimator with the untrained network."""
 import time
 
 import numpy as np
@@ -42,7 +43,9 @@ def test_guided_mean_matches_plain(rx, gpu_ctx, scene):
           f"guided mean {r['sum'].mean() / gn:.5f} plain mean {ref['sum'].mean() / (1 << 18):.5f}")
     assert np.isfinite(r["sum"]).all()
     assert mask.mean() >= 0.95
-    assert frac >= 0.995 and abs(zmean) < 0.1
+    # 24 x 24 x 3 = 1728 values: the count outside 3 sigma is Binomial(1728, 0.0027),
+    # mean 4.7 and sd 2.2 when unbiased; the bar allows 12 (3.3 sd above the mean)
+    assert frac >= 0.993 and abs(zmean) < 0.1
 
 
 # samples per pixel of one render call in the equal-time loops: large enough
@@ -111,6 +114,29 @@ def test_equal_time_relmse_guided_vs_nee_and_plain(rx, gpu_ctx):
     print(f"  guided vs plain: {res['plain'] / res['guided']:.2f}x, guided vs NEE-only: {res['nee'] / res['guided']:.2f}x")
     assert res["guided"] * 2 <= res["plain"]
     assert res["guided"] < res["nee"]
+
+
+def test_equal_time_relmse_glossy(rx, gpu_ctx):
+    """The glossy config (Phong-glossy displaced icosphere, two small area
+    lights) at equal render time, training included: the guided renderer has
+    at least 2x lower relMSE than the plain path tracer against a converged
+    image (NEE-only, 2^15 spp, the same expectation with far less noise);
+    its ratio to NEE alone is reported."""
+    print("[guided] equal-time relMSE, glossy config", flush=True)
+    sc = scenes.icosphere_scene(128, 128, max_depth=10)
+    n_ref = 1 << 15
+    ref = rx.PathTracerHIP(sc, ctx=gpu_ctx).renderPixels(spp=n_ref, mode=MODE_NEE, seed=99)["sum"] / n_ref
+    budget = 2.0
+    res = {}
+    for name, mode in [("plain", MODE_PT), ("nee", MODE_NEE), ("guided", MODE_GUIDED)]:
+        pt = rx.PathTracerHIP(sc, ctx=gpu_ctx)
+        if mode == MODE_GUIDED:
+            pt.guide_init(train_passes=8, batch_size=16384, steps_per_pass=16, learning_rate=1e-2, seed=7)
+        img, n, dt = _equal_time(pt, mode, budget, 64, seed=5)
+        res[name] = rel_mse(img, ref) * dt / budget
+        print(f"  {name:7s}: {n:6d} spp in {dt:.2f} s, relMSE {res[name]:.7f}")
+    print(f"  guided vs plain: {res['plain'] / res['guided']:.2f}x, guided vs NEE-only: {res['nee'] / res['guided']:.2f}x")
+    assert res["guided"] * 2 <= res["plain"]
 
 
 def test_trained_network_beats_untrained(rx, gpu_ctx):
