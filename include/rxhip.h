@@ -74,7 +74,8 @@ enum rx_render_mode {
   RX_MODE_BPT = 3      /* BidirectionalPathTracer (src/BidirectionalPathTracer.cpp:6-398): per pixel and
                           sample one eye and one light subpath, every connection weighted by the reference's
                           balance heuristic; light-tracing contributions land on the pixel
-                          Camera::transToPixel names.  Surface scenes (no RX_OBJ_VPM), max_depth <= 30,
+                          Camera::transToPixel names.  Media (RX_OBJ_VPM) enter through the Ray methods
+                          the reference's connection code calls.  max_depth <= 30,
                           at most 2^22 pixels */
 };
 

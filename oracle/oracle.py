@@ -164,13 +164,10 @@ class SceneOracle:
     def render(self, spp, max_depth=None, rr_start_depth=None, mode=0, seed=1, sample_offset=0, threads=0,
                stats=None, registered=False):
         """FP64 per-pixel (sum, sum of squares), shape (H, W, 3).  mode 0 is the
-        PathTracer, 1 PathTracerTest, 3 the BidirectionalPathTracer (surface
-        scenes only); `registered` (mode 3) lands light-tracing contributions on
+        PathTracer, 1 PathTracerTest, 3 the BidirectionalPathTracer; `registered` (mode 3) lands light-tracing contributions on
         the pixel that covers them instead of the reference's half-pixel-shifted
         one.  If `stats` is a dict it receives the number of traversal queries
         under "rays"."""
-        if mode == 3 and (np.asarray(self.scene.objects["type"]) == 5).any():
-            raise ValueError("the bidirectional tracer takes no participating media")
         cam = self.scene.camera
         H, W = int(cam["height"]), int(cam["width"])
         p = np.zeros((), RENDER_PARAMS)

@@ -470,7 +470,7 @@ static PRay vpm_scatter(Ctx& c, int self, const PRay& in, bool russian) {
       out.insideObject = in.insideObject;
       out.directionSampleType = DEFINITE;
     } else {
-      out.color = out.color * ((current_n * current_n) / (next_n * next_n));
+      if (!c.light) out.color = out.color * ((current_n * current_n) / (next_n * next_n));  // fixIsLight
       out.color = out.color * ((1 - er) / cosine_term(c.sc, out));
       out.directionProb = 1 - p;
       out.contactObject = out.insideObject = self;
@@ -548,7 +548,7 @@ static PRay vpm_scatter(Ctx& c, int self, const PRay& in, bool russian) {
         out.insideObject = in.insideObject;
         out.directionSampleType = DEFINITE;
       } else {
-        out.color = out.color * ((current_n * current_n) / (next_n * next_n));
+        if (!c.light) out.color = out.color * ((current_n * current_n) / (next_n * next_n));  // fixIsLight
         out.color = out.color * ((1 - er) / cosine_term(c.sc, out));
         out.directionProb = 1 - p;
         out.originProb = P_surface(O, in.intersectDist);
@@ -875,9 +875,16 @@ static float self_dir_pdf(const SceneO& sc, const PRay& r) {
   if (r.contactObject < 0) return 0.f;
   return cos_pdf(frame_from(contact_normal(sc, r)), r.direction, 1.f);
 }
-// Ray::getOriginSampleProbDensity (src/Ray.cpp:42-50): 1 unless the ray's
-// inside object is a medium, and media are rejected by this renderer.
-static float origin_pdf(const SceneO&, const PRay&, const PRay&) { return 1.f; }
+// Ray::getOriginSampleProbDensity (src/Ray.cpp:42-50): 1 unless the ray
+// travels in a medium, then SceneVPMObject::getOriginSampleProbDensity
+// (src/SceneVPMObject.cpp:266-273): P_surface for a vertex with a contact
+// object (the camera included), p_medium for a scattering point.
+static float origin_pdf(const SceneO& sc, const PRay& in, const PRay& out) {
+  if (in.insideObject < 0 || sc.objs[in.insideObject].type != OBJ_VPM) return 1.f;
+  const float dist = std::max(length(in.origin - out.origin), EPS);
+  const ObjO& o = sc.objs[in.insideObject];
+  return out.contactObject != NONE ? P_surface(o, dist) : p_medium(o, dist);
+}
 
 // BidirectionalPathTracer::mustUsePT (:124-131)
 static bool must_use_pt(const Path& path) {

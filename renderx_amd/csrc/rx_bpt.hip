@@ -19,6 +19,12 @@
 // Light-tracing contributions (eye prefix of one vertex) land on other
 // pixels: they are added as 2^-32 fixed point with 64-bit integer atomics,
 // so the film stays independent of the order of the additions.
+// VOL: scenes with homogeneous media (SceneVPMObject): subpaths scatter in
+// and cross the media as the path tracer's paths do, and the connection code
+// takes the media's Ray methods: getRadianceDecay (transmittance),
+// getOriginSampleProbDensity (p_medium / P_surface, src/SceneVPMObject.cpp:266-273),
+// the phase function as BSDF and direction density (:257-285) and no
+// cosine term at medium vertices.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -98,7 +104,10 @@ __device__ __forceinline__ Vec3 nrm0(Vec3 v) {
 // a cosine direction about the light normal, colour Le, densities cosine/pi
 // and 1 / (emitting area).  Vertex 0 of both is recorded; eye rays are traced
 // without a condition (src/Camera.cpp:23), light rays with one (:212-223).
+template <bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_bpt_generate(DevScene S, DevCam C, PathBufs B, PassArgs P, BptArgs A) {
+  __shared__ uint2 stack[VOL ? kLdsDepth * kRBlock : 1];
+  uint32_t rays = 0;
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t N = A.N, total = 2 * N;
   if (blockIdx.x == 0)
@@ -112,7 +121,6 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_generate(DevScene S, DevCam C, 
     v.c = v3(1, 1, 1);
     v.dirProb = 1.0f;
     v.originProb = 1.0f;
-    v.inside = NONE;  // Scene::findInsideObject: no volumetric objects here
     v.random = true;
     if (!light) {
       const int x = (int)(pix % (uint32_t)C.W), y = (int)(pix / (uint32_t)C.W);
@@ -123,7 +131,8 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_generate(DevScene S, DevCam C, 
       v.d = normalize(front * C.sight + right * xp + up * yp);
       v.contact = CAMERA;
       v.n = front;  // Camera::getWorldNormal
-      B.s[i] = make_int4(NONE, kNoCondition, -1, 0);
+      v.inside = VOL ? find_inside(S, pos, v.d, NONE, stack + threadIdx.x, B, rays) : NONE;
+      B.s[i] = make_int4(v.inside, kNoCondition, -1, 0);
     } else {
       U4 r = philox(U4{smp, 0xFFFFu, 0u, 1u}, pix, P.key1);
       int lo = 0, hi = S.n_etri - 1;
@@ -146,7 +155,8 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_generate(DevScene S, DevCam C, 
       v.c = dot(nL, w) < 0 ? v3(0, 0, 0) : ld3(S.objs[obj].c1);  // SceneEmissiveObject::getBSDF
       v.dirProb = lobe_pdf(nL, w, 1.0f);
       v.originProb = 1.0f / S.total_area;
-      B.s[i] = make_int4(NONE, obj, -1, 0);
+      v.inside = VOL ? find_inside(S, v.o, w, obj, stack + threadIdx.x, B, rays) : NONE;  // SceneObject::emit (:206)
+      B.s[i] = make_int4(v.inside, obj, -1, 0);
     }
     put_vert(A.V, (size_t)i * A.K, v);
     A.nv[i] = 1;
@@ -155,12 +165,14 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_generate(DevScene S, DevCam C, 
     B.L[i] = make_float4(0, 0, 0, 0);
     B.q0[i] = i;
   }
+  wave_add(&B.stats[0], rays);
 }
 
 // One scatter per queued subpath (samplePathIter's body after its hit is
 // known): the scattered ray is the subpath's next vertex, recorded unless the
 // depth limit replaces it by the (never connectable) terminating ray; a ray
 // that goes on is queued for the next bounce.
+template <bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_bpt_record(DevScene S, PathBufs B, PassArgs P, uint32_t b, BptArgs A) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
   __shared__ uint32_t cont_list[kStageCap], counts[2];
@@ -202,19 +214,11 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_record(DevScene S, PathBufs B, 
       const int obj = inside != NONE ? inside : hitObj;
       if (obj != NONE) {  // otherwise the subpath ends with a terminating ray
         Scat sc;
-        sc.x = o;
-        sc.dir = v3(0, 0, 0);
-        sc.color = v3(1, 1, 1);
-        sc.dirProb = 1.0f;
-        sc.originProb = 1.0f;
-        sc.inside = NONE;
-        sc.contact = NONE;
-        sc.contactPos = -1;
-        sc.random = sc.emitter = sc.transparent = false;
-        int cur = obj;
-        if (S.objs[cur].type == OBJ_REFRACTIVE && hitObj != cur && inside == cur) cur = hitObj;  // delegation
-        if (cur != NONE && pos >= 0)
-          surface_scatter(S, cur, o + d * t, d, hn, pos, inside, russian, u, sc, NONE, light);
+        GuideHead head;
+        float originScale;
+        scatter_at<VOL, false>(S, o, d, t, pos, hitObj, hn, inside, russian, u, lds, B, rays, light, GuideArgs{}, i,
+                               smp, b, pix, P.key1, sc, head, originScale);
+        sc.originProb *= originScale;
         Vec3 nc = v3(0, 0, 0);
         if (sc.contact >= 0 && sc.contactPos >= 0) nc = xyz(S.tree.tn[sc.contactPos]);
         if (length(sc.dir) >= 0.5f && sc.inside == NONE && sc.contact != NONE) {  // light-leak guard (:183-201)
@@ -326,19 +330,22 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_pairs(DevScene S, PathBufs B, B
 
 // The eye vertex of a connection after connectRays (src/MCRenderer.cpp:88-104):
 // turned toward the light vertex, with the inside object that implies.
-__device__ __forceinline__ void eye_connect_state(const BVert& E, const BVert& Eprev, bool has_prev, Vec3 dir,
-                                                  int& inside) {
+template <bool VOL>
+__device__ __forceinline__ void eye_connect_state(const DevScene& S, const BVert& E, const BVert& Eprev, bool has_prev,
+                                                  Vec3 dir, int& inside, uint2* lds, const PathBufs& B,
+                                                  uint32_t& rays) {
   inside = E.inside;
   if (E.contact != NONE && has_prev) {
     if (dot(E.n, dir) <= 0) inside = E.contact;
     else if (dot(E.n, Eprev.d) <= 0) inside = Eprev.inside;
-    else inside = NONE;  // Scene::findInsideObject: no volumetric objects here
+    else inside = VOL ? find_inside(S, E.o, dir, E.contact, lds, B, rays) : NONE;  // Scene::findInsideObject
   }
 }
 
 // MCRenderer::testVisibility for every connectable pair: the first legal hit
 // from the eye vertex (with its connected inside object and its contact
 // object's condition) is not nearer than the light vertex minus EPSILON.
+template <bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_bpt_vis(DevScene S, PathBufs B, BptArgs A) {
   __shared__ uint2 stack[kLdsDepth * kRBlock];
   const uint32_t n = B.cnt[kBptPairCount];
@@ -363,7 +370,7 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_vis(DevScene S, PathBufs B, Bpt
         const BVert L = get_vert(A.V, (size_t)(A.N + p) * A.K + k);
         const Vec3 dir = -nrm0(E.o - L.o);
         int inside;
-        if (j >= 1) eye_connect_state(E, get_vert(A.V, (size_t)p * A.K + j - 1), true, dir, inside);
+        if (j >= 1) eye_connect_state<VOL>(S, E, get_vert(A.V, (size_t)p * A.K + j - 1), true, dir, inside, lds, B, srays);
         else inside = E.inside;
         dist = length(E.o - L.o);
         srays++;
@@ -413,10 +420,11 @@ struct BView {
   }
 };
 
-// Ray::getCosineTerm (src/Ray.cpp:71-78); no medium objects here
-__device__ __forceinline__ float bpt_cos(const BVert& r) {
+// Ray::getCosineTerm (src/Ray.cpp:71-78): media have no cosine term (hasCosineTerm)
+__device__ __forceinline__ float bpt_cos(const DevScene& S, const BVert& r) {
   float d = 1.0f;
-  if (r.contact != NONE && length(r.d) > 0.5f) d = fabsf(dot(r.n, r.d));
+  const bool has = r.contact == CAMERA || (r.contact >= 0 && S.objs[r.contact].type != OBJ_VPM);
+  if (has && length(r.d) > 0.5f) d = fabsf(dot(r.n, r.d));
   return fmaxf(d, 1e-6f);
 }
 // SceneObject material sums / averages of getBSDF and
@@ -425,6 +433,9 @@ __device__ __forceinline__ float bpt_cos(const BVert& r) {
 // src/GlossyMaterial.cpp:59-81; emitters: src/SceneEmissiveObject.cpp:21-26
 __device__ Vec3 obj_bsdf(const DevObject& O, Vec3 in_d, const BVert& out) {
   if (O.type == OBJ_EMISSIVE) return dot(out.n, out.d) < 0 ? v3(0, 0, 0) : ld3(O.c1);
+  // SceneVPMObject::getBSDF (src/SceneVPMObject.cpp:257-264): sigma_s times the phase function at a
+  // scattering point; the dispatch below reaches a medium with a contact only on its own boundary
+  if (O.type == OBJ_VPM) return out.contact == NONE ? ld3(O.c1) * hg(O.g, dot(in_d, out.d)) : v3(0, 0, 0);
   if (out.contact == NONE || O.type == OBJ_REFRACTIVE) return v3(0, 0, 0);
   const Vec3 n = out.n;
   const bool front = dot(n, in_d) <= 0 && dot(n, out.d) >= 0;
@@ -448,6 +459,8 @@ __device__ Vec3 obj_bsdf(const DevObject& O, Vec3 in_d, const BVert& out) {
 }
 __device__ float obj_pdf(const DevObject& O, Vec3 in_d, const BVert& out) {
   if (O.type == OBJ_REFRACTIVE) return 1.0f;  // Material::getDirectionSampleProbDensity
+  // SceneVPMObject::getDirectionSampleProbDensity (:275-285): its own boundary crossings are DEFINITE
+  if (O.type == OBJ_VPM) return (!out.random || out.contact != NONE) ? 0.0f : O.albedo * hg(O.g, dot(in_d, out.d));
   if (O.type == OBJ_EMISSIVE || out.contact == NONE) return 0.0f;
   float pg = 0.0f, pd = 0.0f;
   if (O.type == OBJ_GLOSSY || O.type == OBJ_PHONG) pg = lobe_pdf(mirror(out.n, in_d), out.d, O.expo);
@@ -477,6 +490,15 @@ __device__ __forceinline__ float self_pdf(const BVert& r) {
   return lobe_pdf(r.n, r.d, 1.0f);
 }
 
+// Ray::getOriginSampleProbDensity (src/Ray.cpp:42-50) of the ray `in` for the
+// vertex `out` at distance dist: 1 unless `in` travels in a medium, then
+// P_surface (a surface vertex) or p_medium (a scattering point)
+__device__ __forceinline__ float origin_pdf(const DevScene& S, const BVert& in, const BVert& out, float dist) {
+  if (in.inside < 0 || S.objs[in.inside].type != OBJ_VPM) return 1.0f;
+  const DevObject& O = S.objs[in.inside];
+  return out.contact != NONE ? P_surface(O, dist) : p_medium(O, dist);
+}
+
 __device__ __forceinline__ bool must_use_pt(const BView& v) {
   for (int i = 0; i + 1 < v.W; i++)
     if (v.random(i) && v.random(i + 1)) return false;
@@ -497,8 +519,8 @@ __device__ void connect_color_prob(const DevScene& S, const BView& v, int ci, Ve
       dist = fmaxf(length(prev.o - cur.o), 1e-5f);
       color = color * decay(S, cur.inside, dist);
     }
-    if (i == ci && i < v.W - 1) color = color * (bpt_cos(cur) * bpt_cos(next) / (dist * dist));
-    if (i != ci && i != ci + 1) color = color * bpt_cos(cur);
+    if (i == ci && i < v.W - 1) color = color * (bpt_cos(S, cur) * bpt_cos(S, next) / (dist * dist));
+    if (i != ci && i != ci + 1) color = color * bpt_cos(S, cur);
     prob *= cur.dirProb * cur.originProb;
     color = color * cur.c;
     prev = cur;
@@ -527,11 +549,13 @@ __device__ float connect_weight(const DevScene& S, const BView& v, int ci) {
   for (int i = W - 1; i > 0; i--) {
     c = v.get(i - 1);
     const float dl = fmaxf(length(b.o - c.o), 1e-5f);
-    pb[i - 1] = pb[i] * (double)bpt_cos(link_to(c, b, ci, i - 1, i));
+    pb[i - 1] = pb[i] * (double)bpt_cos(S, link_to(c, b, ci, i - 1, i));
     pb[i - 1] /= (double)dl * (double)dl;
     if (b.random) {
-      if (i < W - 1) pb[i - 1] *= (double)bpt_pdf(S, link_to(a, b, ci, i + 1, i).d, link_to(b, c, ci, i, i - 1));
-      else pb[i - 1] *= (double)self_pdf(link_to(b, c, ci, i, i - 1));
+      const BVert bc = link_to(b, c, ci, i, i - 1);
+      if (i < W - 1) pb[i - 1] *= (double)bpt_pdf(S, link_to(a, b, ci, i + 1, i).d, bc);
+      else pb[i - 1] *= (double)self_pdf(bc);
+      pb[i - 1] *= (double)origin_pdf(S, bc, c, dl);
     }
     a = b;
     b = c;
@@ -548,11 +572,13 @@ __device__ float connect_weight(const DevScene& S, const BView& v, int ci) {
       any_pair = true;
     }
     const float dl = fmaxf(length(c.o - b.o), 1e-5f);
-    double nf = pf * (double)bpt_cos(link_to(c, b, ci, i + 1, i));
+    double nf = pf * (double)bpt_cos(S, link_to(c, b, ci, i + 1, i));
     nf /= (double)dl * (double)dl;
     if (b.random) {
-      if (i > 0) nf *= (double)bpt_pdf(S, link_to(a, b, ci, i - 1, i).d, link_to(b, c, ci, i, i + 1));
-      else nf *= (double)self_pdf(link_to(b, c, ci, i, i + 1));
+      const BVert bc = link_to(b, c, ci, i, i + 1);
+      if (i > 0) nf *= (double)bpt_pdf(S, link_to(a, b, ci, i - 1, i).d, bc);
+      else nf *= (double)self_pdf(bc);
+      nf *= (double)origin_pdf(S, bc, c, dl);
     }
     pf = nf;
     a = b;
@@ -571,7 +597,9 @@ __device__ float connect_weight(const DevScene& S, const BView& v, int ci) {
 // color / prob * weight.  Splits with lp >= 1 are the pair list's, already
 // filtered as the reference's loop filters them (:327-339); the visibility
 // bit is k_bpt_vis's.
-__device__ bool bpt_split(const DevScene& S, const BptArgs& A, uint32_t p, int W, int lp, BView& v, Vec3& out) {
+template <bool VOL>
+__device__ bool bpt_split(const DevScene& S, const BptArgs& A, uint32_t p, int W, int lp, BView& v, Vec3& out,
+                          uint2* lds, const PathBufs& B, uint32_t& rays) {
   const int ep = W - lp, ci = lp - 1;
   v.V = A.V;
   v.eb = (size_t)p * A.K;
@@ -598,12 +626,12 @@ __device__ bool bpt_split(const DevScene& S, const BptArgs& A, uint32_t p, int W
     if (lv.contact != NONE && lp >= 2) {
       if (dot(lv.n, lv.d) <= 0) lv.inside = lv.contact;
       else if (dot(lv.n, pl.d) <= 0) lv.inside = pl.inside;
-      else lv.inside = NONE;
+      else lv.inside = VOL ? find_inside(S, lv.o, lv.d, lv.contact, lds, B, rays) : NONE;
     }
     ev.dirProb = 1.0f;
     ev.c = bpt_bsdf(S, ep >= 2 ? pe.d : ev.d, ev);
     ev.random = true;
-    eye_connect_state(ev, pe, ep >= 2, ev.d, ev.inside);
+    eye_connect_state<VOL>(S, ev, pe, ep >= 2, ev.d, ev.inside, lds, B, rays);
     v.lc = lv;
     v.ec = ev;
   }
@@ -629,15 +657,17 @@ __device__ __forceinline__ void add_fixed(unsigned long long* dst, Vec3 c) {
 
 // Paths that are the eye subpath alone, ending on an emitter (lp = 0): one
 // thread per eye slot, into L[p].
+template <bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_bpt_connect(DevScene S, PathBufs B, BptArgs A) {
   const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t rays = 0;  // lp = 0 joins no vertices: no containment queries
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < A.N; p += stride) {
     const int nE = (int)A.nv[p];
     Vec3 own = v3(0, 0, 0);
     BView v;
     for (int W = 2; W <= nE; W++) {
       Vec3 c;
-      if (bpt_split(S, A, p, W, 0, v, c)) own = own + c;
+      if (bpt_split<VOL>(S, A, p, W, 0, v, c, nullptr, B, rays)) own = own + c;
     }
     B.L[p] = make_float4(own.x, own.y, own.z, 0.0f);
   }
@@ -649,8 +679,11 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_connect(DevScene S, PathBufs B,
 // go to the pixel Camera::transToPixel gives the light vertex
 // (src/Camera.cpp:136-152), scaled by the light-tracing factors (:357-373),
 // into the pass's splat image of the slot's sample.
+template <bool VOL>
 __global__ void __launch_bounds__(kRBlock) k_bpt_connect_pairs(DevScene S, DevCam C, PathBufs B, PassArgs P,
                                                                BptArgs A) {
+  __shared__ uint2 stack[VOL ? kLdsDepth * kRBlock : 1];
+  uint32_t rays = 0;
   const uint32_t n = B.cnt[kBptPairCount];
   const uint32_t stride = gridDim.x * blockDim.x;
   const Vec3 front = ld3(C.front), right = ld3(C.right), up = ld3(C.up), cpos = ld3(C.pos);
@@ -661,7 +694,7 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_connect_pairs(DevScene S, DevCa
     const int ep = (int)j + 1, lp = (int)k + 1;
     BView v;
     Vec3 color;
-    if (!bpt_split(S, A, p, ep + lp, lp, v, color)) continue;
+    if (!bpt_split<VOL>(S, A, p, ep + lp, lp, v, color, stack + threadIdx.x, B, rays)) continue;
     if (ep >= 2) {
       add_fixed(A.own + 3 * (size_t)p, color);
       continue;
@@ -694,6 +727,7 @@ __global__ void __launch_bounds__(kRBlock) k_bpt_connect_pairs(DevScene S, DevCa
     }
     if (ix >= 0) add_fixed(A.splat + 3 * ((size_t)jsmp * npix + (uint32_t)(iy * C.W + ix)), color);
   }
+  wave_add(&B.stats[0], rays);
 }
 
 // Iteration images in sample order: own + splatted radiance of sample j,
@@ -740,8 +774,6 @@ static __global__ void k_fill_u32(uint32_t* a, size_t n) {
 int render_bpt(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm, float* rgb_mean, double* sum,
                double* sumsq, rx_render_stats* stats) {
   RenderState& R = *c->render;
-  for (const DevObject& o : R.scene.host_objs)
-    if (o.type == OBJ_VPM) return set_err(c, RX_ERR_INVALID, "rx_render: the bidirectional tracer takes no participating media");
   if (prm->max_depth + 2 > (uint32_t)kBptMaxVerts)
     return set_err(c, RX_ERR_INVALID, "rx_render: the bidirectional tracer needs max_depth <= 30");
   const uint32_t npix = (uint32_t)(cam->width * cam->height);
@@ -836,6 +868,7 @@ int render_bpt(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm
   P.rr_start = prm->rr_start_depth;
   P.mode = prm->mode;
   const DevScene S = R.scene.dev;
+  const bool vol = S.n_vol > 0;  // kernels without the medium code when the scene has no volumes
   const BatchArgs BA0{};
   hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, B.sum, 3 * (size_t)npix);
   hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, B.sumsq, 3 * (size_t)npix);
@@ -853,16 +886,24 @@ int render_bpt(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, A.vis, (size_t)A.N * K);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, reinterpret_cast<double*>(A.splat), 3 * (size_t)A.N);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, reinterpret_cast<double*>(A.own), 3 * (size_t)A.N);
-    hipLaunchKernelGGL(k_bpt_generate, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, A);
+    if (vol) hipLaunchKernelGGL(k_bpt_generate<true>, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, A);
+    else hipLaunchKernelGGL(k_bpt_generate<false>, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, A);
     for (uint32_t b = 0; b <= prm->max_depth; b++) {
       hipLaunchKernelGGL((k_extend<true, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, b, BA0);
-      hipLaunchKernelGGL(k_bpt_record, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, A);
+      if (vol) hipLaunchKernelGGL(k_bpt_record<true>, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, A);
+      else hipLaunchKernelGGL(k_bpt_record<false>, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, A);
       launches++;
     }
     hipLaunchKernelGGL(k_bpt_pairs, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
-    hipLaunchKernelGGL(k_bpt_vis, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
-    hipLaunchKernelGGL(k_bpt_connect, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
-    hipLaunchKernelGGL(k_bpt_connect_pairs, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, A);
+    if (vol) {
+      hipLaunchKernelGGL(k_bpt_vis<true>, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
+      hipLaunchKernelGGL(k_bpt_connect<true>, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
+      hipLaunchKernelGGL(k_bpt_connect_pairs<true>, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, A);
+    } else {
+      hipLaunchKernelGGL(k_bpt_vis<false>, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
+      hipLaunchKernelGGL(k_bpt_connect<false>, dim3(blocks), dim3(kRBlock), 0, s, S, B, A);
+      hipLaunchKernelGGL(k_bpt_connect_pairs<false>, dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, A);
+    }
     hipLaunchKernelGGL(k_bpt_accumulate, dim3(blocks), dim3(kRBlock), 0, s, B, P, C,
                        (const unsigned long long*)A.splat, (const unsigned long long*)A.own);
     RXCK(hipGetLastError(), "launch pass");

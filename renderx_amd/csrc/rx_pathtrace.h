@@ -848,6 +848,179 @@ __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3
 #ifndef RX_SHADE_MIN_WAVES
 #define RX_SHADE_MIN_WAVES 6
 #endif
+// One scatter of samplePathIter (src/MCRenderer.cpp:151-267) after the hit
+// of the ray (o, d) is known (t, pos, hitObj, normal hn; inside: the object
+// the ray travels in): the object resolution of the level loop (a medium
+// the ray is in, a surface inside it, delegation from a refractive object to
+// the surface it hits) and that object's scatter.  originScale receives the
+// surface-inside-a-medium density the caller multiplies into originProb.
+// GUIDED: the guided scatter of guided vertices (row GA.row[i], random
+// numbers of (smp, b, pix, key1)), whose mixture head is left in `head`.
+// light: the scatter of a light subpath (fixIsLight).
+template <bool VOL, bool GUIDED>
+__device__ __forceinline__ void scatter_at(const DevScene& S, Vec3 o, Vec3 d, float t, int pos, int hitObj, Vec3 hn,
+                                           int inside, bool russian, const float* u, uint2* lds, const PathBufs& B,
+                                           uint32_t& rays, bool light, const GuideArgs& GA, uint32_t i, uint32_t smp,
+                                           uint32_t b, uint32_t pix, uint32_t key1, Scat& sc, GuideHead& head,
+                                           float& originScale) {
+  const int obj = inside != NONE ? inside : hitObj;
+  sc.dir = v3(0, 0, 0);
+  sc.color = v3(1, 1, 1);
+  sc.dirProb = 1.0f;
+  sc.originProb = 1.0f;
+  sc.inside = NONE;
+  sc.contact = NONE;
+  sc.contactPos = -1;
+  sc.random = sc.emitter = sc.transparent = false;
+  sc.neeKind = 0;
+  sc.neeObj = NONE;
+  sc.pdfB = 0.0f;
+  sc.guided = false;
+  sc.pmix = sc.pb = sc.qrr = 0.0f;
+  const Vec3 xh = o + d * t;  // hit point (valid when pos >= 0)
+  // the one containment query a scatter can need: leaving the object
+  // the path is in through its own boundary (a refractive object, or a
+  // medium whose sampled distance passes the boundary)
+  int exit_outside = NONE;
+  if (VOL && pos >= 0 && inside != NONE && hitObj == inside) {
+    const DevObject& I = S.objs[inside];
+    if (I.type == OBJ_REFRACTIVE || (I.type == OBJ_VPM && -__logf(1.0f - u[5]) / I.lum_st >= t))
+      exit_outside = find_inside(S, xh, d, inside, lds, B, rays);
+  }
+  int cur = obj;
+  originScale = 1.0f;
+  bool done = false;
+  for (int level = 0; level < 3 && !done; level++) {
+    const DevObject& O = S.objs[cur];
+    if (VOL && O.type == OBJ_VPM) {  // SceneVPMObject::scatter (src/SceneVPMObject.cpp:4-251)
+      const float sampDist = -__logf(1.0f - u[5]) / O.lum_st;
+      const bool go_in = (hitObj == cur) && (inside != cur);
+      const bool be_in = (inside == cur);
+      const bool out_of_vol = !(pos >= 0) || sampDist >= t;
+      if (go_in || (be_in && out_of_vol && hitObj == cur)) {
+        // cross the own boundary, entering or leaving
+        int outside = go_in ? cur : exit_outside;
+        float n_i = go_in ? refr_coeff(S, inside) : O.ior, n_t = refr_coeff(S, outside);
+        sc.x = xh;
+        sc.contact = cur;
+        sc.contactPos = pos;
+        sc.random = false;
+        if (be_in) sc.originProb = P_surface(O, t);
+        Vec3 tr;
+        float er;
+        if (!refract(d, hn, n_i, n_t, tr, er)) {
+          sc.dir = v3(0, 0, 0);
+          sc.contact = NONE;
+          sc.color = v3(0, 0, 0);
+        } else {
+          float pr = fminf(fmaxf(er, 0.0f), 1.0f);
+          if (u[4] < pr) {  // medium boundaries carry no cosine term
+            sc.dir = normalize(mirror(hn, d));
+            sc.color = v3(er, er, er);
+            sc.dirProb = pr;
+            sc.inside = inside;
+          } else {
+            float f = (light ? 1.0f : (n_i * n_i) / (n_t * n_t)) * (1 - er);  // fixIsLight (:84,219)
+            sc.dir = tr;
+            sc.color = v3(f, f, f);
+            sc.dirProb = 1 - pr;
+            sc.inside = outside;
+            sc.transparent = (n_i == n_t);
+          }
+        }
+        done = true;
+      } else if (be_in && !out_of_vol) {  // scatter inside the medium
+        sc.x = o + d * sampDist;
+        // guided mode: with probability alpha the learned mixture proposes the direction,
+        // otherwise the phase function does; one-sample MIS over both densities
+        const bool gm = GUIDED && GA.row[i] != kNoRow;
+        float ug[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+        if (gm) {
+          guide_head(GA.head + (size_t)GA.row[i] * kGHead, head);
+          U4 r3 = philox(U4{smp, b, 3u, 0u}, pix, key1);
+          ug[0] = u01(r3.x);
+          ug[1] = u01(r3.y);
+          ug[2] = u01(r3.z);
+          ug[3] = u01(r3.w);
+        }
+        const float g = O.g;
+        Vec3 w;
+        if (gm && ug[0] < head.alpha) {
+          w = guide_sample(head, ug[1], ug[2], ug[3]);
+        } else {
+          Frame lf = frame_from(d);
+          float cosT;
+          if (fabsf(g) < 1e-5f) cosT = 1 - 2 * u[6];
+          else {
+            float sq = (1 - g * g) / (1 - g + 2 * g * u[6]);
+            cosT = (1 + g * g - sq * sq) / (2 * g);
+          }
+          float sinT = sqrtf(fmaxf(0.0f, 1 - cosT * cosT));
+          float sp, cp;
+          __sincosf(6.28318530717958647692f * u[7], &sp, &cp);
+          w = normalize(to_world(lf, v3(sinT * cp, cosT, sinT * sp)));
+        }
+        float ph = hg(g, dot(d, w));
+        sc.inside = cur;
+        sc.contact = NONE;
+        sc.random = true;
+        sc.neeKind = 2;
+        sc.neeObj = cur;
+        sc.originProb = p_medium(O, sampDist);
+        float pdir = ph;
+        if (gm) {
+          pdir = head.alpha * guide_pdf(head, w) + (1.0f - head.alpha) * ph;
+          sc.guided = true;
+          sc.pmix = pdir;
+          sc.pb = ph;
+          sc.qrr = russian ? O.albedo : 1.0f;
+        }
+        if ((u[2] < O.albedo || !russian) && pdir > 0.0f) {
+          sc.dir = w;
+          sc.color = ld3(O.c1) * ph;
+          sc.dirProb = russian ? O.albedo * pdir : pdir;
+          sc.pdfB = sc.dirProb;
+        } else {
+          sc.dir = v3(0, 0, 0);
+          sc.color = v3(0, 0, 0);
+        }
+        done = true;
+      } else if (be_in && out_of_vol && pos >= 0) {  // a surface inside the medium
+        originScale = P_surface(O, t);
+        cur = hitObj;
+      } else {  // inside a medium with no hit: nothing to scatter
+        sc.dir = v3(0, 0, 0);
+        sc.color = v3(0, 0, 0);
+        sc.x = o;
+        done = true;
+      }
+    } else if (O.type == OBJ_REFRACTIVE && hitObj != cur && inside == cur) {
+      if (pos < 0) {
+        sc.x = o;
+        done = true;
+      } else {
+        cur = hitObj;  // RefractiveMaterial delegates to the hit object
+      }
+    } else {
+      if (pos < 0) {
+        sc.x = o;
+        done = true;
+      } else if (GUIDED && dot(d, hn) <= 0 &&
+                 (S.objs[cur].type == OBJ_DIFFUSE || S.objs[cur].type == OBJ_GLOSSY ||
+                  S.objs[cur].type == OBJ_PHONG) && GA.row[i] != kNoRow) {
+        guide_head(GA.head + (size_t)GA.row[i] * kGHead, head);
+        U4 r3 = philox(U4{smp, b, 3u, 0u}, pix, key1);
+        const float ug[4] = {u01(r3.x), u01(r3.y), u01(r3.z), u01(r3.w)};
+        guided_scatter(S.objs[cur], cur, xh, d, hn, pos, inside, russian, head, ug, u[2], sc);
+        done = true;
+      } else {
+        surface_scatter(S, cur, xh, d, hn, pos, inside, russian, u, sc, exit_outside, light);
+        done = true;
+      }
+    }
+  }
+}
+
 template <int MODE>
 struct ShadeLds {
   uint2 stack[kLdsDepth * kRBlock];
@@ -902,162 +1075,10 @@ __device__ __forceinline__ void shade_body(DevScene S0, PathBufs B, PassArgs P, 
       const int obj = inside != NONE ? inside : hitObj;  // MCRenderer::samplePathIter :169-181
       if (obj != NONE) {
         Scat sc;
-        sc.dir = v3(0, 0, 0);
-        sc.color = v3(1, 1, 1);
-        sc.dirProb = 1.0f;
-        sc.originProb = 1.0f;
-        sc.inside = NONE;
-        sc.contact = NONE;
-        sc.contactPos = -1;
-        sc.random = sc.emitter = sc.transparent = false;
-        sc.neeKind = 0;
-        sc.neeObj = NONE;
-        sc.pdfB = 0.0f;
-        sc.guided = false;
-        sc.pmix = sc.pb = sc.qrr = 0.0f;
         GuideHead head;
-        const Vec3 xh = o + d * t;  // hit point (valid when pos >= 0)
-        // the one containment query a scatter can need: leaving the object
-        // the path is in through its own boundary (a refractive object, or a
-        // medium whose sampled distance passes the boundary)
-        int exit_outside = NONE;
-        if (VOL && pos >= 0 && inside != NONE && hitObj == inside) {
-          const DevObject& I = S.objs[inside];
-          if (I.type == OBJ_REFRACTIVE || (I.type == OBJ_VPM && -__logf(1.0f - u[5]) / I.lum_st >= t))
-            exit_outside = find_inside(S, xh, d, inside, lds, B, rays);
-        }
-        int cur = obj;
-        float originScale = 1.0f;
-        bool done = false;
-        for (int level = 0; level < 3 && !done; level++) {
-          const DevObject& O = S.objs[cur];
-          if (VOL && O.type == OBJ_VPM) {  // SceneVPMObject::scatter (src/SceneVPMObject.cpp:4-251)
-            const float sampDist = -__logf(1.0f - u[5]) / O.lum_st;
-            const bool go_in = (hitObj == cur) && (inside != cur);
-            const bool be_in = (inside == cur);
-            const bool out_of_vol = !(pos >= 0) || sampDist >= t;
-            if (go_in || (be_in && out_of_vol && hitObj == cur)) {
-              // cross the own boundary, entering or leaving
-              int outside = go_in ? cur : exit_outside;
-              float n_i = go_in ? refr_coeff(S, inside) : O.ior, n_t = refr_coeff(S, outside);
-              sc.x = xh;
-              sc.contact = cur;
-              sc.contactPos = pos;
-              sc.random = false;
-              if (be_in) sc.originProb = P_surface(O, t);
-              Vec3 tr;
-              float er;
-              if (!refract(d, hn, n_i, n_t, tr, er)) {
-                sc.dir = v3(0, 0, 0);
-                sc.contact = NONE;
-                sc.color = v3(0, 0, 0);
-              } else {
-                float pr = fminf(fmaxf(er, 0.0f), 1.0f);
-                if (u[4] < pr) {  // medium boundaries carry no cosine term
-                  sc.dir = normalize(mirror(hn, d));
-                  sc.color = v3(er, er, er);
-                  sc.dirProb = pr;
-                  sc.inside = inside;
-                } else {
-                  float f = (n_i * n_i) / (n_t * n_t) * (1 - er);
-                  sc.dir = tr;
-                  sc.color = v3(f, f, f);
-                  sc.dirProb = 1 - pr;
-                  sc.inside = outside;
-                  sc.transparent = (n_i == n_t);
-                }
-              }
-              done = true;
-            } else if (be_in && !out_of_vol) {  // scatter inside the medium
-              sc.x = o + d * sampDist;
-              // guided mode: with probability alpha the learned mixture proposes the direction,
-              // otherwise the phase function does; one-sample MIS over both densities
-              const bool gm = MODE == 2 && GA.row[i] != kNoRow;
-              float ug[4] = {1.0f, 0.0f, 0.0f, 0.0f};
-              if (gm) {
-                guide_head(GA.head + (size_t)GA.row[i] * kGHead, head);
-                U4 r3 = philox(U4{smp, b, 3u, 0u}, pix, pc.key1);
-                ug[0] = u01(r3.x);
-                ug[1] = u01(r3.y);
-                ug[2] = u01(r3.z);
-                ug[3] = u01(r3.w);
-              }
-              const float g = O.g;
-              Vec3 w;
-              if (gm && ug[0] < head.alpha) {
-                w = guide_sample(head, ug[1], ug[2], ug[3]);
-              } else {
-                Frame lf = frame_from(d);
-                float cosT;
-                if (fabsf(g) < 1e-5f) cosT = 1 - 2 * u[6];
-                else {
-                  float sq = (1 - g * g) / (1 - g + 2 * g * u[6]);
-                  cosT = (1 + g * g - sq * sq) / (2 * g);
-                }
-                float sinT = sqrtf(fmaxf(0.0f, 1 - cosT * cosT));
-                float sp, cp;
-                __sincosf(6.28318530717958647692f * u[7], &sp, &cp);
-                w = normalize(to_world(lf, v3(sinT * cp, cosT, sinT * sp)));
-              }
-              float ph = hg(g, dot(d, w));
-              sc.inside = cur;
-              sc.contact = NONE;
-              sc.random = true;
-              sc.neeKind = 2;
-              sc.neeObj = cur;
-              sc.originProb = p_medium(O, sampDist);
-              float pdir = ph;
-              if (gm) {
-                pdir = head.alpha * guide_pdf(head, w) + (1.0f - head.alpha) * ph;
-                sc.guided = true;
-                sc.pmix = pdir;
-                sc.pb = ph;
-                sc.qrr = russian ? O.albedo : 1.0f;
-              }
-              if ((u[2] < O.albedo || !russian) && pdir > 0.0f) {
-                sc.dir = w;
-                sc.color = ld3(O.c1) * ph;
-                sc.dirProb = russian ? O.albedo * pdir : pdir;
-                sc.pdfB = sc.dirProb;
-              } else {
-                sc.dir = v3(0, 0, 0);
-                sc.color = v3(0, 0, 0);
-              }
-              done = true;
-            } else if (be_in && out_of_vol && pos >= 0) {  // a surface inside the medium
-              originScale = P_surface(O, t);
-              cur = hitObj;
-            } else {  // inside a medium with no hit: nothing to scatter
-              sc.dir = v3(0, 0, 0);
-              sc.color = v3(0, 0, 0);
-              sc.x = o;
-              done = true;
-            }
-          } else if (O.type == OBJ_REFRACTIVE && hitObj != cur && inside == cur) {
-            if (pos < 0) {
-              sc.x = o;
-              done = true;
-            } else {
-              cur = hitObj;  // RefractiveMaterial delegates to the hit object
-            }
-          } else {
-            if (pos < 0) {
-              sc.x = o;
-              done = true;
-            } else if (MODE == 2 && dot(d, hn) <= 0 &&
-                       (S.objs[cur].type == OBJ_DIFFUSE || S.objs[cur].type == OBJ_GLOSSY ||
-                        S.objs[cur].type == OBJ_PHONG) && GA.row[i] != kNoRow) {
-              guide_head(GA.head + (size_t)GA.row[i] * kGHead, head);
-              U4 r3 = philox(U4{smp, b, 3u, 0u}, pix, pc.key1);
-              const float ug[4] = {u01(r3.x), u01(r3.y), u01(r3.z), u01(r3.w)};
-              guided_scatter(S.objs[cur], cur, xh, d, hn, pos, inside, russian, head, ug, u[2], sc);
-              done = true;
-            } else {
-              surface_scatter(S, cur, xh, d, hn, pos, inside, russian, u, sc, exit_outside);
-              done = true;
-            }
-          }
-        }
+        float originScale;
+        scatter_at<VOL, MODE == 2>(S, o, d, t, pos, hitObj, hn, inside, russian, u, lds, B, rays, false, GA, i, smp, b,
+                                   pix, pc.key1, sc, head, originScale);
         sc.originProb *= originScale;
         // segment into the new vertex: decay by the incoming ray's medium, 1/originProb
         const float seg = length(sc.x - o);

@@ -193,6 +193,24 @@ def lit_plane(width=32, height=32, spp=64, max_depth=8, rr_start_depth=0) -> Sce
     return Scene("lit_plane", v, t, o, cam, spp, max_depth, rr_start_depth)
 
 
+def fog_room(width=16, height=16, spp=64, max_depth=30, rr_start_depth=0, sigma_s=0.3, sigma_a=0.05, g=0.5) -> Scene:
+    """The Cornell box (diffuse walls and boxes, quad light) inside a
+    homogeneous, index-matched medium that also holds the camera.  Every
+    vertex of a path that can still reach the light lies in the medium, so
+    every connection of a bidirectional tracer can exist and its image has the
+    path tracer's expectation (up to the depth limits: max_depth 30 with
+    Russian roulette at every bounce leaves paths that long a negligible
+    share).  Diffuse only: a Phong lobe pick would separate the two
+    estimators."""
+    b = _Builder()
+    _cornell(b)
+    mv, mt = _axis_box([-0.6, -0.6, -1.8], [1.6, 1.6, 1.6])
+    b.add(mv, mt, VPM, (sigma_s, sigma_s, sigma_s), 0.0, (sigma_a, sigma_a, sigma_a), 1.0, g)
+    v, t, o = b.finish()
+    cam = make_camera([0.5, 0.5, -1.374], [0.5, 0.5, 0.5], [0, 1, 0], width, height)
+    return Scene("fog_room", v, t, o, cam, spp, max_depth, rr_start_depth)
+
+
 def absorbing_slab(width=8, height=8, spp=64, max_depth=8, sigma_a=1.0, thickness=0.5, le=1.0) -> Scene:
     """A purely absorbing, index-matched medium slab (sigma_s = 0, IOR 1) between
     the camera and a large emitter that fills the view.  Every camera ray

@@ -55,7 +55,26 @@ def test_reference_landing_pixel_doubles_first_column(oracle_mod):
     assert abs(inner_u - inner_r) < 0.1 * inner_r
 
 
-def test_bpt_rejects_media(oracle_mod):
-    sc = scenes.media_scene(8, 8, freq=4, glass_freq=2)
-    with pytest.raises(ValueError, match="media"):
-        oracle_mod.SceneOracle(sc).render(1, mode=3)
+def test_registered_bpt_in_fog_matches_pathtracer(oracle_mod):
+    """A homogeneous medium around everything (camera and light included):
+    subpaths scatter in it and every connection carries its transmittance and
+    phase function, so the registered image matches the PathTracer's
+    (PathTracerTest's next-event connections carry no transmittance in a
+    medium, so the plain PathTracer is the reference here).  The image mean
+    cannot see the distance densities in the balance weights (weights formed
+    from the same densities sum to one whatever they are); those densities are
+    the medium densities that test_oracle_pins pins."""
+    sc = scenes.fog_room(12, 12)
+    o = oracle_mod.SceneOracle(sc)
+    nb, npt = 1024, 16384
+    sb, qb = o.render(nb, mode=3, seed=51, registered=True)
+    sp, qp = o.render(npt, mode=0, seed=52)
+    mask = clt_mask(sb, qb, sp, qp)
+    frac, zmean, z = three_sigma(sb, qb, nb, sp, qp, npt, pooled=False, mask=mask)
+    mb, mp = sb.mean() / nb, sp.mean() / npt
+    print(f"  fog: within 3 sigma {frac:.4f}, mean z {zmean:+.4f}, excluded {1 - mask.mean():.4f}, "
+          f"means bpt {mb:.5f} pt {mp:.5f}")
+    assert np.isfinite(sb).all()
+    assert mask.mean() >= 0.95
+    assert frac >= 0.99 and abs(zmean) < 0.15
+    assert abs(mb - mp) < 0.02 * mp

@@ -6,7 +6,11 @@ mean z near 0, image means within 4 standard errors; on the Cornell box and
 on a Cornell box with a glass sphere (caustics through specular chains, light
 subpaths refracting without the eye path's (n_i/n_t)^2).  With the
 registered landing pixel the image is checked against a long GPU PathTracer
-render instead, an estimator that shares none of the connection code.  Plus
+render instead, an estimator that shares none of the connection code.  On
+the media scene and on a Cornell box inside a medium that holds the camera
+and the light, the subpaths scatter in a homogeneous medium and the
+connections take its transmittance, phase function and distance densities.
+Plus
 determinism, independence from the pass size, and the rejections."""
 import math
 
@@ -28,7 +32,21 @@ GPU_SPP = 1 << 18
 CASES = {
     "cornell": (lambda: scenes.cornell(16, 16, spp=16, max_depth=8, rr_start_depth=0), 16384),
     "glass": (lambda: scenes.glass_scene(16, 16, max_depth=8), 8192),
+    # the media scene (homogeneous medium filling the box, glass sphere, Phong
+    # sphere): subpaths scatter in the medium, connections inside it carry its
+    # transmittance, phase function and p_medium / P_surface densities
+    "media": (lambda: scenes.media_scene(16, 16, max_depth=6, freq=12, glass_freq=6), 32768),
+    # the Cornell box inside a medium that also holds the camera and the light
+    "fog": (lambda: scenes.fog_room(16, 16), 16384),
 }
+# In the two media scenes light subpaths scatter anywhere in the volume, and
+# scattering points close to the camera splat large light-traced values (the
+# 1/d^2 of the camera connection); with boundary crossings and glass in the
+# media scene as well, their per-pixel distributions are heavier-tailed than
+# the surface scenes' (up to 3 % of the media scene's pixel-channels fail the
+# CLT mask at these sample counts): bars of 98.5 % within 3 sigma and
+# |mean z| < 0.15 there.
+BARS = {"media": (0.985, 0.15), "fog": (0.985, 0.15)}
 _oracle_cache = {}
 
 
@@ -71,7 +89,8 @@ def test_image_matches_oracle_bpt(rx, oracle_mod, gpu_ctx, name):
     sc, osum, osq, on = _oracle(oracle_mod, name)
     r = rx.BidirectionalPathTracerHIP(sc, ctx=gpu_ctx).renderPixels(spp=GPU_SPP, seed=11)
     assert r["stats"]["samples"] == 16 * 16 * GPU_SPP
-    _check(name, r["sum"], r["sumsq"], GPU_SPP, osum, osq, on, frac_min=0.99)
+    frac_min, zmax = BARS.get(name, (0.99, 0.1))
+    _check(name, r["sum"], r["sumsq"], GPU_SPP, osum, osq, on, frac_min=frac_min, zmax=zmax)
 
 
 def test_registered_matches_gpu_pathtracer(rx, gpu_ctx):
@@ -108,9 +127,6 @@ def test_deterministic_and_pass_size_invariant(rx, gpu_ctx):
 
 
 def test_rejections(rx, gpu_ctx):
-    media = scenes.media_scene(8, 8, freq=4, glass_freq=2)
-    with pytest.raises(rx.RxError, match="media"):
-        rx.BidirectionalPathTracerHIP(media, ctx=gpu_ctx).renderPixels(spp=1)
     sc = scenes.cornell(8, 8)
     with pytest.raises(rx.RxError, match="max_depth"):
         rx.BidirectionalPathTracerHIP(sc, ctx=gpu_ctx).renderPixels(spp=1, max_depth=31)
