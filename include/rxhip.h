@@ -114,7 +114,9 @@ typedef struct rx_render_tuning {
                              with block_size 0/128; deeper entries spill to a per-thread global stack */
   uint32_t block_size;    /* threads per block of the extend kernel: 0 (default 128), 64 or 256 (with the default
                              stack_depth); the grid keeps the default shape's thread count */
-  uint32_t reserved;
+  uint32_t camera_rays;   /* rx_render's camera rays: 0 (default) traced as packets of 64 (a wave walks the tree
+                             once for the samples of neighbouring pixels), 1 one ray per lane like the later
+                             bounces; the hits are identical */
 } rx_render_tuning;
 
 typedef struct rx_render_stats {
@@ -249,8 +251,10 @@ int rx_render(rx_ctx* ctx, const rx_camera_desc* camera, const rx_render_params*
 int rx_set_render_tuning(rx_ctx* ctx, const rx_render_tuning* tuning);
 int rx_get_render_tuning(rx_ctx* ctx, rx_render_tuning* tuning);
 /* Times rx_render with each candidate launch shape on `params` (spp capped
- * at params->spp) and keeps the fastest in the context; `best` (may be NULL)
- * receives it and `best_ms` its time.  No counterpart in the reference. */
+ * at params->spp) and keeps the fastest in the context: path slots x blocks
+ * per CU, then the extend kernel's stack depth and block size, then camera
+ * rays per lane instead of in packets.  `best` (may be NULL) receives it and
+ * `best_ms` its time.  No counterpart in the reference. */
 int rx_autotune_render(rx_ctx* ctx, const rx_camera_desc* camera, const rx_render_params* params,
                        rx_render_tuning* best, float* best_ms);
 

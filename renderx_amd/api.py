@@ -27,16 +27,16 @@ RENDER_STATS = np.dtype([("rays", "<u8"), ("extend_rays", "<u8"), ("shadow_rays"
                          ("extend_launches", "<u8"), ("guide_rows", "<u8"), ("guide_train_rows", "<u8")])
 RENDER_TIMING, RENDER_COUNT_WORK = 1, 2
 RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use_graph", "<u4"), ("stack_depth", "<u4"),
-                          ("block_size", "<u4"), ("reserved", "<u4")])
+                          ("block_size", "<u4"), ("camera_rays", "<u4")])
 GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("batch_size", "<u4"), ("steps_per_pass", "<u4"), ("learning_rate", "<f4"),
                          ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4")])
 MODE_PT, MODE_NEE, MODE_GUIDED, MODE_BPT = 0, 1, 2, 3
 
 
-def _tuning(path_slots, blocks_per_cu, use_graph, stack_depth, block_size):
+def _tuning(path_slots, blocks_per_cu, use_graph, stack_depth, block_size, camera_rays=0):
     t = np.zeros(1, RENDER_TUNING)
     t["path_slots"], t["blocks_per_cu"], t["use_graph"] = path_slots, blocks_per_cu, use_graph
-    t["stack_depth"], t["block_size"] = stack_depth, block_size
+    t["stack_depth"], t["block_size"], t["camera_rays"] = stack_depth, block_size, camera_rays
     return t
 GUIDE_LR_DECAY = 1
 
@@ -160,16 +160,16 @@ class PathTracerHIP:
         p["seed"] = seed
         self.ctx.check(lib().rx_guide_init(self.ctx.h, ptr(p)), "rx_guide_init")
 
-    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0, stack_depth=0, block_size=0):
+    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0, stack_depth=0, block_size=0, camera_rays=0):
         """Launch shape of renderPixels (rx_render_tuning); images do not depend on it."""
         self.ctx.check(lib().rx_set_render_tuning(self.ctx.h, ptr(_tuning(path_slots, blocks_per_cu, use_graph,
-                                                                          stack_depth, block_size))),
+                                                                          stack_depth, block_size, camera_rays))),
                        "rx_set_render_tuning")
 
     def get_tuning(self):
         t = np.zeros(1, RENDER_TUNING)
         self.ctx.check(lib().rx_get_render_tuning(self.ctx.h, ptr(t)), "rx_get_render_tuning")
-        return {k: int(t[k][0]) for k in RENDER_TUNING.names if k != "reserved"}
+        return {k: int(t[k][0]) for k in RENDER_TUNING.names}
 
     def autotune(self, spp=None, max_depth=None, rr_start_depth=None, mode=MODE_PT, seed=1):
         """rx_autotune_render: time the candidate launch shapes, keep the fastest."""
@@ -182,7 +182,7 @@ class PathTracerHIP:
         t = np.zeros(1, RENDER_TUNING)
         ms = np.zeros(1, np.float32)
         self.ctx.check(lib().rx_autotune_render(self.ctx.h, ptr(cam), ptr(p), ptr(t), ptr(ms)), "rx_autotune_render")
-        return {k: int(t[k][0]) for k in RENDER_TUNING.names if k != "reserved"}, float(ms[0])
+        return {k: int(t[k][0]) for k in RENDER_TUNING.names}, float(ms[0])
 
     def renderPixels(self, camera=None, spp=None, max_depth=None, rr_start_depth=None, mode=MODE_PT, seed=1,
                      sample_offset=0, want_sums=True, flags=0):
@@ -266,7 +266,7 @@ class BatchRenderer:
         t = np.zeros(1, RENDER_TUNING)
         ms = np.zeros(1, np.float32)
         self.ctx.check(lib().rx_batch_autotune(self.h, passes, ptr(t), ptr(ms)), "rx_batch_autotune")
-        return {k: int(t[k][0]) for k in RENDER_TUNING.names if k != "reserved"}, float(ms[0])
+        return {k: int(t[k][0]) for k in RENDER_TUNING.names}, float(ms[0])
 
     def render(self):
         st = np.zeros(1, RENDER_STATS)

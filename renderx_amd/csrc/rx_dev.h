@@ -556,6 +556,111 @@ __device__ __forceinline__ void trace_range(const DevTree& T, uint32_t n, uint32
   }
 }
 
+// ------------------------------------------------------------------ packets
+// Nearest hit of up to 64 coherent rays, one per lane (camera rays of the
+// same pixels), traced as one packet: the wave walks the tree once, node by
+// node, carrying the 64-bit mask of the lanes whose rays reach the node.  At
+// an interior node every lane of the mask classifies both child boxes as the
+// per-lane traversal does (filter, exact predicate when undecided, culling by
+// its own best distance); the children that some lane enters are visited,
+// the one more lanes find nearer first, the other pushed with its mask on a
+// wave stack in LDS.  At a leaf the lanes of the mask test its triangles.
+// A lane thus visits every subtree whose boxes pass for its ray and are not
+// culled by its own best hit, so its result is the per-lane traversal's: the
+// winner rule (nearest, ties to the larger leaf, first triangle in a leaf)
+// does not depend on the visiting order.  Node, leaf and triangle addresses
+// are wave-uniform, so one fetch serves the packet.  Every lane of the wave
+// runs this function (invalid lanes with valid = false); `wstack` is the
+// wave's max_sp entries (a node and its mask per entry).
+template <bool STATS>
+__device__ __forceinline__ void trace_packet(const DevTree& T, const TreeRoot& root, bool valid, Vec3 o, Vec3 d,
+                                             uint4* wstack, int max_sp, TraceState& s, TraceStats* st,
+                                             bool* overflow) {
+  const float INF = __builtin_inff();
+  const int lane = threadIdx.x & 63;
+  const bool in_root = valid && trace_begin<STATS>(root, s, o, d, -1, kNoCondition, st);  // no condition: ids unused
+  unsigned long long mask = __ballot(in_root);
+  int32_t node = root.ref;
+  int sp = 0;
+  for (;;) {
+    while (mask && node >= 0) {
+      const float4* np = T.nodes + 4 * (size_t)node;
+      const float4 a = np[0], b = np[1], c = np[2];
+      const int4 r = *reinterpret_cast<const int4*>(np + 3);
+      bool vl = false, vr = false;
+      float tl = 0.0f, tr = 0.0f;
+      if ((mask >> lane) & 1ull) {
+        const float llo[3] = {a.x, a.y, a.z}, lhi[3] = {a.w, b.x, b.y};
+        const float rlo[3] = {b.z, b.w, c.x}, rhi[3] = {c.y, c.z, c.w};
+        if (STATS) st->boxes += 2;
+        const float thr = s.has ? s.best * 1.001f : INF;
+        bool hl, hr;
+        if (s.fast) {
+          const int cl = slab_classify(llo, lhi, s.o, s.inv, s.beta, tl);
+          const int cr = slab_classify(rlo, rhi, s.o, s.inv, s.beta, tr);
+          hl = cl == kBoxHit || (cl == kBoxUnknown && tl <= thr && box_hit(llo, lhi, s.o, s.d));
+          hr = cr == kBoxHit || (cr == kBoxUnknown && tr <= thr && box_hit(rlo, rhi, s.o, s.d));
+        } else {
+          hl = box_hit(llo, lhi, s.o, s.d);
+          hr = box_hit(rlo, rhi, s.o, s.d);
+          tl = slab_tnear(llo, lhi, s.o, s.d, s.inv);
+          tr = slab_tnear(rlo, rhi, s.o, s.d, s.inv);
+        }
+        vl = hl && tl <= thr;
+        vr = hr && tr <= thr;
+      }
+      const unsigned long long ML = __ballot(vl), MR = __ballot(vr);
+      const int32_t left = __builtin_amdgcn_readfirstlane(r.x), right = __builtin_amdgcn_readfirstlane(r.y);
+      if (ML && MR) {
+        const unsigned long long preferR = __ballot(vl && vr && tr < tl);
+        const bool rightFirst = 2 * __popcll(preferR) > __popcll(ML & MR);
+        const unsigned long long farM = rightFirst ? ML : MR;
+        if (sp >= max_sp) {  // deeper than the wave stack: the caller reports an overflow
+          *overflow = true;
+          s.has = false;
+          return;
+        }
+        wstack[sp] = make_uint4((uint32_t)(rightFirst ? left : right), (uint32_t)farM, (uint32_t)(farM >> 32), 0u);
+        sp++;
+        node = rightFirst ? right : left;
+        mask = rightFirst ? MR : ML;
+      } else if (ML) {
+        node = left;
+        mask = ML;
+      } else if (MR) {
+        node = right;
+        mask = MR;
+      } else {
+        node = INT32_MIN;
+      }
+    }
+    if (mask && node != INT32_MIN) {  // a leaf: its triangles for the lanes of the mask
+      const int32_t leaf = ~node;
+      const int32_t p0 = __builtin_amdgcn_readfirstlane(T.leaf_start[leaf]);
+      const int32_t p1 = __builtin_amdgcn_readfirstlane(T.leaf_start[leaf + 1]);
+      const bool act = (mask >> lane) & 1ull;
+      for (int32_t p = p0; p < p1; p++) {
+        const float4 ta = T.tv0[p], tb = T.tv1[p], tc = T.tv2[p];
+        if (act) {
+          if (STATS) st->tris++;
+          const float dist = tri_dist(xyz(ta), xyz(tb), xyz(tc), s.o, s.d);
+          if (dist >= 0 && (!s.has || dist < s.best || (dist == s.best && leaf > s.bestLeaf))) {
+            s.has = true;
+            s.best = dist;
+            s.bestLeaf = leaf;
+            s.bestPos = p;
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    sp--;
+    const uint4 e = wstack[sp];
+    node = __builtin_amdgcn_readfirstlane((int32_t)e.x);
+    mask = ((unsigned long long)__builtin_amdgcn_readfirstlane(e.z) << 32) | __builtin_amdgcn_readfirstlane(e.y);
+  }
+}
+
 // ------------------------------------------------------------------ RNG
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter-based, one
 // independent stream per (pixel, sample) keyed by the seed.

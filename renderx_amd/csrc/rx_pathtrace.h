@@ -470,6 +470,45 @@ __global__ void __launch_bounds__(BLOCK, kExtendMinWaves) k_extend(DevTree T, Pa
   extend_body<LEG, STATS, BATCH, DEPTH>(T, B, b, BA, stack);
 }
 
+// Camera rays (bounce 0, no condition) as packets of 64: a wave traces the
+// 64 neighbouring queue entries of its chunk (with pixel-major slots, the
+// samples of the same pixels) as one packet (trace_packet), its first chunk
+// by its wave index, the others from the bounce's work counter.  The hits are
+// the per-lane kernel's.
+constexpr int kPacketDepth = 64;  // wave stack entries: trees up to 64 levels
+static __global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend_packet(DevTree T, PathBufs B, uint32_t b) {
+  __shared__ uint4 wstack[kRBlock / 64][kPacketDepth];
+  const uint32_t n = B.cnt[b];
+  const uint32_t* q = (b & 1) ? B.q1 : B.q0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  uint32_t* ctr = B.cnt + kWorkBase + b * kWorkWords;
+  uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (uint32_t)wv) * 64u;
+  bool ovf = false;
+  const TreeRoot root = tree_root(T);
+  while (base < n) {  // wave-uniform
+    const uint32_t i = base + (uint32_t)lane;
+    const bool valid = i < n;
+    const uint32_t p = valid ? q[i] : 0u;
+    Vec3 o = v3(0, 0, 0), d = v3(0, 0, 1);
+    if (valid) {
+      o = xyz(B.o[p]);
+      d = xyz(B.d[p]);
+    }
+    TraceState ts;
+    trace_packet<false>(T, root, valid, o, d, wstack[wv], kPacketDepth, ts, nullptr, &ovf);
+    if (valid) B.hit[p] = ts.has ? make_int2(__float_as_int(ts.best), ts.bestPos) : make_int2(__float_as_int(-1.0f), -1);
+    uint32_t next = 0;
+    if (lane == 0) next = waves * 64u + atomicAdd(ctr, 64u);
+    base = __builtin_amdgcn_readfirstlane(next);
+  }
+  if (ovf) *B.overflow = 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(&B.stats[0], (unsigned long long)n);
+    atomicAdd(&B.stats[1], (unsigned long long)n);
+  }
+}
+
 // Launch shapes of the extend kernel: (LDS stack depth, block size) pairs
 // with an instantiation.  Every shape gives the same hits; the grid keeps
 // the thread count of the default shape (blocks * kRBlock threads), which

@@ -306,6 +306,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     GA.rec = R.rec.as<float>();
     GA.R = Rs;
   }
+  // camera rays as packets unless the tuning asks for lanes or the tree is deeper than the packet stack
+  const bool packets = c->tuning.camera_rays == 0 && R.scene.max_depth < (uint32_t)kPacketDepth;
   std::vector<hipEvent_t> evs;
   auto mark = [&](void) -> hipEvent_t {
     hipEvent_t e;
@@ -340,6 +342,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       const uint32_t sdep = c->tuning.stack_depth, sblk = c->tuning.block_size;
       if (b == 0) {
         if (count) launch_extend<false, true, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
+        else if (packets) hipLaunchKernelGGL(k_extend_packet, dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, b);
         else launch_extend<false, false, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
       } else {
         if (count) launch_extend<true, true, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
@@ -452,6 +455,7 @@ int rx_set_render_tuning(rx_ctx* c, const rx_render_tuning* t) {
     return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: path_slots must be 0 or in [4096, 2^27]");
   if (t->blocks_per_cu > 64) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: blocks_per_cu > 64");
   if (t->use_graph > 2) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: use_graph is 0, 1 or 2");
+  if (t->camera_rays > 1) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: camera_rays is 0 or 1");
   if (!extend_shape_ok(t->stack_depth, t->block_size))
     return set_err(c, RX_ERR_INVALID,
                    "rx_set_render_tuning: (stack_depth, block_size) must be (0|10, 0|64|128|256) or (6|8|12|16, 0|128)");
@@ -509,12 +513,18 @@ int rx_autotune_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_par
       if (int r = trial(t)) return r;
     }
   }
-  const rx_render_tuning base = win;
+  rx_render_tuning base = win;
   for (const ExtendShape& e : kExtendShapes) {
     if (e.depth == kLdsDepth && e.block == kRBlock) continue;  // timed above
     rx_render_tuning t = base;
     t.stack_depth = (uint32_t)e.depth;
     t.block_size = (uint32_t)e.block;
+    if (int r = trial(t)) return r;
+  }
+  base = win;  // then camera rays one per lane instead of in packets
+  {
+    rx_render_tuning t = base;
+    t.camera_rays = 1;
     if (int r = trial(t)) return r;
   }
   c->tuning = win;

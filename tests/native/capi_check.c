@@ -25,7 +25,7 @@ _Static_assert(sizeof(rx_render_params) == 32 && offsetof(rx_render_params, seed
                    offsetof(rx_render_params, flags) == 28,
                "rx_render_params layout");
 _Static_assert(sizeof(rx_render_tuning) == 24 && offsetof(rx_render_tuning, stack_depth) == 12 &&
-                   offsetof(rx_render_tuning, block_size) == 16,
+                   offsetof(rx_render_tuning, block_size) == 16 && offsetof(rx_render_tuning, camera_rays) == 20,
                "rx_render_tuning layout");
 _Static_assert(sizeof(rx_render_stats) == 112 && offsetof(rx_render_stats, ms_total) == 56 &&
                    offsetof(rx_render_stats, extend_launches) == 88 && offsetof(rx_render_stats, guide_rows) == 96,

@@ -182,6 +182,26 @@ def test_absorbing_slab_closed_form(rx, gpu_ctx, mode):
     assert zmax < 5.0 and zmean < 4.0
 
 
+def test_camera_packets_full_size(rx, gpu_ctx):
+    """Camera rays traced as packets of 64 (the default) and one per lane hit
+    the same triangles at the same distances: on the 1,003,526-triangle
+    icosphere scene at 1024x1024 the two films are bit-identical, and the ray
+    counts agree."""
+    print("[packets] icosphere 1M tris, 1024^2", flush=True)
+    sc = scenes.icosphere_scene(1024, 1024, max_depth=10)
+    pt = rx.PathTracerHIP(sc, ctx=gpu_ctx)
+    try:
+        pt.set_tuning(camera_rays=0)
+        a = pt.renderPixels(spp=4, mode=MODE_PT, seed=21)
+        pt.set_tuning(camera_rays=1)
+        b = pt.renderPixels(spp=4, mode=MODE_PT, seed=21)
+    finally:
+        pt.set_tuning()
+    print(f"  packets {a['stats']['ms_total']:.2f} ms, lanes {b['stats']['ms_total']:.2f} ms")
+    assert a["stats"]["rays"] == b["stats"]["rays"]
+    assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sumsq"], b["sumsq"])
+
+
 def test_launch_tuning_gives_identical_images(rx, gpu_ctx):
     """rx_render_tuning (samples per pass, resident blocks, the extend
     kernel's stack depth and block size) and the autotuner change only the
@@ -198,6 +218,9 @@ def test_launch_tuning_gives_identical_images(rx, gpu_ctx):
             pt.set_tuning(path_slots=slots, blocks_per_cu=blk, stack_depth=depth, block_size=bsz)
             b = pt.renderPixels(spp=48, mode=MODE_NEE, seed=3)
             assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sumsq"], b["sumsq"]), (slots, blk, depth, bsz)
+        pt.set_tuning(camera_rays=1)  # camera rays one per lane instead of in packets
+        b = pt.renderPixels(spp=48, mode=MODE_NEE, seed=3)
+        assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sumsq"], b["sumsq"])
         for depth, bsz in [(7, 0), (16, 256), (0, 32)]:  # shapes without an instantiation
             with pytest.raises(rx.RxError):
                 pt.set_tuning(stack_depth=depth, block_size=bsz)
