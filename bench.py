@@ -223,6 +223,25 @@ def guided_profile(res=512, calls=4, chunk=64):
     return out
 
 
+def batch64_profile():
+    """The 64-scene batch rendered once with graph replay at the default
+    shape, after a warm-up render (for a kernel trace or counters of the
+    batch kernels).  Returns the render's stats."""
+    from renderx_amd import api, scenes
+
+    scs = [scenes.soup_scene(i) for i in range(64)]
+    br = api.BatchRenderer(ctx=api.Context(0))
+    for k, sc in enumerate(scs):
+        br.add(sc, mode=api.MODE_PT, seed=1000 + k)
+    br.set_tuning(use_graph=1)
+    br.render()
+    st = br.render()
+    log(f"[bench] batch64 profile render: {st['ms_total']:.0f} ms, {st['samples'] / 1e6:.0f} M samples, "
+        f"{st['rays'] / 1e6:.0f} M rays, {st['extend_launches']} extend launches")
+    return {"ms": st["ms_total"], "samples": int(st["samples"]), "rays": int(st["rays"]),
+            "extend_rays": int(st["extend_rays"]), "extend_launches": int(st["extend_launches"])}
+
+
 def batch64(steps=2):
     """The 64-scene triangle-soup batch (10^4..2*10^6 triangles per scene,
     128^2..1024^2 pixels, 4..64 spp, max depth 8): all scenes rendered as one
@@ -328,12 +347,17 @@ def main():
                     help="after the run, copy this many MiB device-to-device (reference kernel for PMC byte counters)")
     ap.add_argument("--guided-profile", action="store_true",
                     help="only the guided render of the equal-time study (for a kernel trace); prints its JSON")
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="only one graph-replayed render of the 64-scene batch (for a kernel trace); prints its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only"}))
         sys.exit(2)
     if args.guided_profile:
         print(json.dumps(guided_profile()), flush=True)
+        return
+    if args.batch_profile:
+        print(json.dumps(batch64_profile()), flush=True)
         return
 
     from renderx_amd import api, scenes

@@ -114,9 +114,11 @@ typedef struct rx_render_tuning {
                              with block_size 0/128; deeper entries spill to a per-thread global stack */
   uint32_t block_size;    /* threads per block of the extend kernel: 0 (default 128), 64 or 256 (with the default
                              stack_depth); the grid keeps the default shape's thread count */
-  uint32_t camera_rays;   /* rx_render's camera rays: 0 (default) traced as packets of 64 (a wave walks the tree
-                             once for the samples of neighbouring pixels), 1 one ray per lane like the later
-                             bounces; the hits are identical */
+  uint32_t camera_rays;   /* camera rays: 1 one ray per lane like the later bounces, 2 packets of 64 (a wave walks
+                             the tree once for neighbouring samples), 0 the default: packets for rx_render (a
+                             wave holds the samples of two pixels), lanes for batches (a wave holds 64
+                             neighbouring pixels of a scene, too far apart to share a walk).  The hits are
+                             identical (the persistent batch launch traces them per lane) */
 } rx_render_tuning;
 
 typedef struct rx_render_stats {
@@ -281,8 +283,8 @@ int rx_batch_render(rx_batch* batch, rx_render_stats* stats);
  * candidate launch shape and keeps the fastest (time per camera path) as the
  * context's rx_render_tuning: path slots (the wavefront size, which is also
  * every queue's capacity) x blocks per CU with graph replay, then the extend
- * kernel's stack depth and block size, then stream launches and the one
- * persistent launch.  `best` and `best_ms` (ms per camera path) may be NULL.
+ * kernel's stack depth and block size, then camera rays per lane, then
+ * stream launches and the one persistent launch.  `best` and `best_ms` (ms per camera path) may be NULL.
  * No counterpart in the reference. */
 int rx_batch_autotune(rx_batch* batch, uint32_t passes, rx_render_tuning* best, float* best_ms);
 /* One scene's film as rx_render returns it (any output may be NULL). */

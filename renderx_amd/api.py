@@ -250,12 +250,13 @@ class BatchRenderer:
         self.shapes.append((int(cam["height"][0]), int(cam["width"][0])))
         return int(idx.value)
 
-    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0, stack_depth=0, block_size=0):
+    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0, stack_depth=0, block_size=0, camera_rays=0):
         """Launch shape (rx_render_tuning of the context); images do not depend on it.
         use_graph: 0 stream launches, 1 one pass recorded in a hipGraph and
-        replayed, 2 the whole batch as one persistent cooperative launch."""
+        replayed, 2 the whole batch as one persistent cooperative launch;
+        camera_rays: 0 default, 1 one per lane, 2 packets of 64."""
         self.ctx.check(lib().rx_set_render_tuning(self.ctx.h, ptr(_tuning(path_slots, blocks_per_cu, use_graph,
-                                                                          stack_depth, block_size))),
+                                                                          stack_depth, block_size, camera_rays))),
                        "rx_set_render_tuning")
 
     def autotune(self, passes=2):

@@ -356,10 +356,15 @@ int batch_render(rx_batch* b, rx_render_stats* stats, uint32_t max_passes) {
   hipLaunchKernelGGL(k_pass_set, dim3(1), dim3(64), 0, s, BA.pass_id, 0u);
   RXCK(hipGetLastError(), "launch fill");
   const uint32_t sdep = c->tuning.stack_depth, sblk = c->tuning.block_size;
+  // camera rays one per lane unless the tuning asks for packets: a batch pass holds one sample per
+  // pixel, so a wave's 64 rays are 64 neighbouring pixels, whose walks share too little
+  const bool packets = c->tuning.camera_rays == 2 && b->max_tree_depth < (uint32_t)kPacketDepth;
   auto launch_pass = [&]() {
     hipLaunchKernelGGL((k_generate<true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, C0, B, P, BA);
     for (uint32_t bb = 0; bb <= Dmax; bb++) {
-      if (bb == 0) launch_extend<false, false, true>(sdep, sblk, blocks, s, S.tree, B, bb, BA);
+      if (bb == 0 && packets)
+        hipLaunchKernelGGL(k_extend_packet<true>, dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, bb, BA);
+      else if (bb == 0) launch_extend<false, false, true>(sdep, sblk, blocks, s, S.tree, B, bb, BA);
       else launch_extend<true, false, true>(sdep, sblk, blocks, s, S.tree, B, bb, BA);
       if (b->mode == RX_MODE_NEE) {
         hipLaunchKernelGGL((k_shade<1, true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
@@ -487,11 +492,17 @@ int rx_batch_autotune(rx_batch* b, uint32_t passes, rx_render_tuning* best, floa
     t.block_size = (uint32_t)e.block;
     if (int r = trial(t)) return r;
   }
+  base = win;  // camera rays in packets instead of one per lane
+  {
+    rx_render_tuning t = base;
+    t.camera_rays = 2;
+    if (int r = trial(t)) return r;
+  }
   base = win;
   for (uint32_t g : {0u, 2u}) {
     rx_render_tuning t = base;
     t.use_graph = g;
-    if (g == 2) t.stack_depth = t.block_size = 0;  // the persistent kernel has the default extend shape
+    if (g == 2) t.stack_depth = t.block_size = t.camera_rays = 0;  // the persistent kernel has the default shape
     if (int r = trial(t)) return r;
   }
   c->tuning = win;

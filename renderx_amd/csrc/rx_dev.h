@@ -571,14 +571,17 @@ __device__ __forceinline__ void trace_range(const DevTree& T, uint32_t n, uint32
 // does not depend on the visiting order.  Node, leaf and triangle addresses
 // are wave-uniform, so one fetch serves the packet.  Every lane of the wave
 // runs this function (invalid lanes with valid = false); `wstack` is the
-// wave's max_sp entries (a node and its mask per entry).
-template <bool STATS>
+// wave's max_sp entries (a node and its mask per entry).  LEGALITY: each
+// lane's hits must pass its ray's condition (inside, contact), as in
+// trace_trip (packets of later bounces hit the same triangles but, their
+// directions scattered, walk the union of their paths: slower than lanes).
+template <bool STATS, bool LEGALITY>
 __device__ __forceinline__ void trace_packet(const DevTree& T, const TreeRoot& root, bool valid, Vec3 o, Vec3 d,
-                                             uint4* wstack, int max_sp, TraceState& s, TraceStats* st,
-                                             bool* overflow) {
+                                             int32_t inside, int32_t contact, uint4* wstack, int max_sp,
+                                             TraceState& s, TraceStats* st, bool* overflow) {
   const float INF = __builtin_inff();
   const int lane = threadIdx.x & 63;
-  const bool in_root = valid && trace_begin<STATS>(root, s, o, d, -1, kNoCondition, st);  // no condition: ids unused
+  const bool in_root = valid && trace_begin<STATS>(root, s, o, d, inside, contact, st);
   unsigned long long mask = __ballot(in_root);
   int32_t node = root.ref;
   int sp = 0;
@@ -644,7 +647,8 @@ __device__ __forceinline__ void trace_packet(const DevTree& T, const TreeRoot& r
         if (act) {
           if (STATS) st->tris++;
           const float dist = tri_dist(xyz(ta), xyz(tb), xyz(tc), s.o, s.d);
-          if (dist >= 0 && (!s.has || dist < s.best || (dist == s.best && leaf > s.bestLeaf))) {
+          if (dist >= 0 && (!s.has || dist < s.best || (dist == s.best && leaf > s.bestLeaf)) &&
+              (!LEGALITY || legal(s.d, xyz(T.tn[p]), __float_as_int(tb.w), s.inside, s.contact))) {
             s.has = true;
             s.best = dist;
             s.bestLeaf = leaf;

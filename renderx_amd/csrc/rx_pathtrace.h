@@ -472,11 +472,15 @@ __global__ void __launch_bounds__(BLOCK, kExtendMinWaves) k_extend(DevTree T, Pa
 
 // Camera rays (bounce 0, no condition) as packets of 64: a wave traces the
 // 64 neighbouring queue entries of its chunk (with pixel-major slots, the
-// samples of the same pixels) as one packet (trace_packet), its first chunk
-// by its wave index, the others from the bounce's work counter.  The hits are
-// the per-lane kernel's.
+// samples of the same pixels; in a batch, neighbouring pixels of a scene) as
+// one packet (trace_packet), its first chunk by its wave index, the others
+// from the bounce's work counter.  BATCH: the lanes of a chunk that belong
+// to different scenes (at a segment boundary) form one packet per scene,
+// each from its scene's root.  The hits are the per-lane kernel's.
 constexpr int kPacketDepth = 64;  // wave stack entries: trees up to 64 levels
-static __global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend_packet(DevTree T, PathBufs B, uint32_t b) {
+template <bool BATCH>
+__global__ void __launch_bounds__(kRBlock, kExtendMinWaves)
+    k_extend_packet(DevTree T, PathBufs B, uint32_t b, BatchArgs BA) {
   __shared__ uint4 wstack[kRBlock / 64][kPacketDepth];
   const uint32_t n = B.cnt[b];
   const uint32_t* q = (b & 1) ? B.q1 : B.q0;
@@ -485,19 +489,38 @@ static __global__ void __launch_bounds__(kRBlock, kExtendMinWaves) k_extend_pack
   uint32_t* ctr = B.cnt + kWorkBase + b * kWorkWords;
   uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (uint32_t)wv) * 64u;
   bool ovf = false;
-  const TreeRoot root = tree_root(T);
   while (base < n) {  // wave-uniform
     const uint32_t i = base + (uint32_t)lane;
     const bool valid = i < n;
     const uint32_t p = valid ? q[i] : 0u;
     Vec3 o = v3(0, 0, 0), d = v3(0, 0, 1);
+    uint32_t scn = 0;
     if (valid) {
       o = xyz(B.o[p]);
       d = xyz(B.d[p]);
+      if (BATCH) scn = BA.scn[p];
     }
-    TraceState ts;
-    trace_packet<false>(T, root, valid, o, d, wstack[wv], kPacketDepth, ts, nullptr, &ovf);
-    if (valid) B.hit[p] = ts.has ? make_int2(__float_as_int(ts.best), ts.bestPos) : make_int2(__float_as_int(-1.0f), -1);
+    bool pending = valid;
+    for (unsigned long long left = __ballot(pending); left; left = __ballot(pending)) {  // a packet per scene
+      const uint32_t cur = BATCH ? (uint32_t)__shfl((int)scn, __ffsll((long long)left) - 1) : 0u;
+      const bool mine = pending && (!BATCH || scn == cur);
+      TreeRoot root = tree_root(T);
+      if (BATCH) {
+        const SceneInfo& I = BA.info[cur];
+        root.ref = I.root_ref;
+        for (int a = 0; a < 3; a++) {
+          root.lo[a] = I.root_lo[a];
+          root.hi[a] = I.root_hi[a];
+        }
+      }
+      TraceState ts;
+      trace_packet<false, false>(T, root, mine, o, d, -1, kNoCondition, wstack[wv], kPacketDepth, ts, nullptr,
+                                 &ovf);
+      if (mine) {
+        B.hit[p] = ts.has ? make_int2(__float_as_int(ts.best), ts.bestPos) : make_int2(__float_as_int(-1.0f), -1);
+        pending = false;
+      }
+    }
     uint32_t next = 0;
     if (lane == 0) next = waves * 64u + atomicAdd(ctr, 64u);
     base = __builtin_amdgcn_readfirstlane(next);

@@ -307,7 +307,9 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     GA.R = Rs;
   }
   // camera rays as packets unless the tuning asks for lanes or the tree is deeper than the packet stack
-  const bool packets = c->tuning.camera_rays == 0 && R.scene.max_depth < (uint32_t)kPacketDepth;
+  // (the later bounces' rays leave a pixel's hit points in scattered directions: packets of them walk
+  // the union of their paths through the tree and are slower than one ray per lane)
+  const bool packets = c->tuning.camera_rays != 1 && R.scene.max_depth < (uint32_t)kPacketDepth;
   std::vector<hipEvent_t> evs;
   auto mark = [&](void) -> hipEvent_t {
     hipEvent_t e;
@@ -342,7 +344,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       const uint32_t sdep = c->tuning.stack_depth, sblk = c->tuning.block_size;
       if (b == 0) {
         if (count) launch_extend<false, true, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
-        else if (packets) hipLaunchKernelGGL(k_extend_packet, dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, b);
+        else if (packets) hipLaunchKernelGGL(k_extend_packet<false>, dim3(blocks), dim3(kRBlock), 0, s, S.tree, B, b, BA0);
         else launch_extend<false, false, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
       } else {
         if (count) launch_extend<true, true, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
@@ -455,7 +457,7 @@ int rx_set_render_tuning(rx_ctx* c, const rx_render_tuning* t) {
     return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: path_slots must be 0 or in [4096, 2^27]");
   if (t->blocks_per_cu > 64) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: blocks_per_cu > 64");
   if (t->use_graph > 2) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: use_graph is 0, 1 or 2");
-  if (t->camera_rays > 1) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: camera_rays is 0 or 1");
+  if (t->camera_rays > 2) return set_err(c, RX_ERR_INVALID, "rx_set_render_tuning: camera_rays is 0, 1 or 2");
   if (!extend_shape_ok(t->stack_depth, t->block_size))
     return set_err(c, RX_ERR_INVALID,
                    "rx_set_render_tuning: (stack_depth, block_size) must be (0|10, 0|64|128|256) or (6|8|12|16, 0|128)");

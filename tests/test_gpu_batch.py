@@ -33,9 +33,11 @@ def _scene_set():
     return out
 
 
-# (path slots, extend stack depth, extend block size); 4096 slots split the
-# larger scenes over several passes
-SHAPES = [(0, 0, 0), (4096, 0, 0), (0, 8, 128), (4096, 16, 128), (0, 10, 256), (0, 10, 64)]
+# (path slots, extend stack depth, extend block size, camera rays: 0 the
+# batch default (one per lane), 2 in packets); 4096 slots split the larger
+# scenes over several passes
+SHAPES = [(0, 0, 0, 0), (4096, 0, 0, 0), (0, 8, 128, 0), (4096, 16, 128, 0), (0, 10, 256, 0), (0, 10, 64, 0),
+          (0, 0, 0, 2)]
 
 
 @pytest.mark.parametrize("mode", [MODE_PT, MODE_NEE])
@@ -51,10 +53,10 @@ def test_batch_images_equal_single_renders(rx, gpu_ctx, mode, graph):
     for k, sc in enumerate(scs):
         assert br.add(sc, mode=mode, seed=100 + k) == k
     try:
-        for slots, depth, block in SHAPES:
-            if graph == 2 and (depth or block):
+        for slots, depth, block, lanes in SHAPES:
+            if graph == 2 and (depth or block or lanes):
                 continue  # the persistent launch has the default extend shape
-            pt.set_tuning(path_slots=slots, use_graph=graph, stack_depth=depth, block_size=block)
+            pt.set_tuning(path_slots=slots, use_graph=graph, stack_depth=depth, block_size=block, camera_rays=lanes)
             st = br.render()
             assert st["samples"] == sum(sc.camera["width"] * sc.camera["height"] * sc.spp for sc in scs)
             for k in range(len(scs)):
@@ -63,7 +65,7 @@ def test_batch_images_equal_single_renders(rx, gpu_ctx, mode, graph):
                 assert np.array_equal(img["sum"], a["sum"]), (k, slots)
                 assert np.array_equal(img["sumsq"], a["sumsq"]), (k, slots)
                 assert np.array_equal(img["mean"], a["mean"]), (k, slots)
-            print(f"  slots={slots} depth={depth} block={block}: {len(scs)} scenes identical, {st['rays']} rays "
+            print(f"  slots={slots} depth={depth} block={block} lanes={lanes}: {len(scs)} scenes identical, {st['rays']} rays "
                   f"in {st['ms_total']:.2f} ms")
     finally:
         pt.set_tuning()

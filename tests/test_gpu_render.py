@@ -191,7 +191,7 @@ def test_camera_packets_full_size(rx, gpu_ctx):
     sc = scenes.icosphere_scene(1024, 1024, max_depth=10)
     pt = rx.PathTracerHIP(sc, ctx=gpu_ctx)
     try:
-        pt.set_tuning(camera_rays=0)
+        pt.set_tuning(camera_rays=2)
         a = pt.renderPixels(spp=4, mode=MODE_PT, seed=21)
         pt.set_tuning(camera_rays=1)
         b = pt.renderPixels(spp=4, mode=MODE_PT, seed=21)
