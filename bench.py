@@ -32,11 +32,12 @@ HBM_PEAK_GBS = 8000.0
 B_RAY = 44
 B_BOX = 32
 B_TRI = 40
-# HBM bytes per k_extend launch of this workload, measured by two rocprofv3
-# --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on the
-# rx_copy_kernel 256 MiB copy; the commands and raw figures are in
-# profiles/k_extend_icosphere1M_1024px.md
-EXTEND_TRAFFIC_BYTES = 1841.76e6
+# HBM bytes per extend launch of this workload (the camera rays' packet
+# kernel and the later bounces' per-lane kernel, 1 + 10 launches per pass),
+# measured by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the
+# same counters on the rx_copy_kernel 256 MiB copy; the commands and raw
+# figures are in profiles/k_extend_icosphere1M_1024px.md
+EXTEND_TRAFFIC_BYTES = 1194.2e6
 
 
 def log(*a):
@@ -72,12 +73,65 @@ def cpu_baseline(scene, seconds_target=15.0):
     spp_f = min(64, scale)
     st = {}
     t = time.perf_counter()
-    so.render(spp_f, stats=st)
+    so.render(spp_f, stats=st, threads=cores)
     dt = time.perf_counter() - t
+    out = {"value": st["rays"] / dt, "unit": "rays/s", "cores": cores, "kind": "port",
+           "sample": f"oracle PathTracer on the same scene at {res}x{res} px, {spp_f} spp, max depth "
+                     f"{scene.max_depth} ({st['rays']} traversal queries in {dt:.1f} s)"}
+    # one thread, on a sample of about a sixth of the time budget
+    cam["width"] = cam["height"] = 128
+    cam["sight_dist"] = int(round(64 / np.tan(np.radians(20.0))))
+    so.set_camera(cam.copy())
+    st1 = {}
+    t = time.perf_counter()
+    so.render(max(1, spp_f // 4), stats=st1, threads=1)
+    dt1 = time.perf_counter() - t
+    so.render(1, threads=cores)  # restore the thread count for later oracle calls
     del so
-    return {"value": st["rays"] / dt, "unit": "rays/s", "cores": cores, "kind": "port",
-            "sample": f"oracle PathTracer on the same scene at {res}x{res} px, {spp_f} spp, max depth "
-                      f"{scene.max_depth} ({st['rays']} traversal queries in {dt:.1f} s)"}
+    single = {"value": st1["rays"] / dt1, "unit": "rays/s", "cores": 1, "kind": "port",
+              "sample": f"oracle PathTracer on the same scene at 128x128 px, {max(1, spp_f // 4)} spp "
+                        f"({st1['rays']} traversal queries in {dt1:.1f} s)"}
+    return out, single
+
+
+def cornell_config0(spp=16, cpu=True):
+    """The reference's own CPU-runnable case: the Cornell box (36 triangles)
+    at 256x256, 16 spp, max depth 8, Russian roulette after depth 3, plain
+    path tracing; GPU render time and the oracle's on all host threads and
+    on one."""
+    from renderx_amd import api, scenes
+
+    sc = scenes.cornell(256, 256, spp=spp, max_depth=8, rr_start_depth=3)
+    pt = api.PathTracerHIP(sc, device=0)
+    pt.renderPixels(spp=spp, mode=api.MODE_PT, seed=1, want_sums=False)  # warm-up
+    t = time.perf_counter()
+    reps = 20
+    ms = 0.0
+    rays = 0
+    for k in range(reps):
+        st = pt.renderPixels(spp=spp, mode=api.MODE_PT, seed=2 + k, want_sums=False)["stats"]
+        ms += st["ms_total"]
+        rays += st["rays"]
+    wall = time.perf_counter() - t
+    out = {"scene": "Cornell box 36 tris, 256x256 px, 16 spp, max depth 8, RR after depth 3, plain PT",
+           "gpu_ms_per_render": ms / reps, "gpu_wall_ms_per_render": wall / reps * 1e3,
+           "gpu_samples_per_s": 256 * 256 * spp * reps / wall, "gpu_rays_per_s": rays / wall}
+    log(f"[bench] config 0 Cornell: GPU {ms / reps:.2f} ms per render ({wall / reps * 1e3:.2f} ms wall)")
+    if cpu:
+        from oracle import oracle
+
+        so = oracle.SceneOracle(sc)
+        cores = int(oracle.lib().orc_max_threads())
+        for threads, n in [(cores, spp), (1, 2)]:
+            st = {}
+            t = time.perf_counter()
+            so.render(n, stats=st, threads=threads)
+            dt = time.perf_counter() - t
+            out[f"cpu_{threads}_threads_s_per_render"] = dt * spp / n
+            out[f"cpu_{threads}_threads_rays_per_s"] = st["rays"] / dt
+            log(f"[bench] config 0 Cornell: oracle on {threads} threads {dt * spp / n:.2f} s per render")
+        so.render(1, threads=cores)
+    return out
 
 
 def equal_time_relmse(res=512, ref_spp=65536, budget_s=5.0, chunk=64):
@@ -420,7 +474,7 @@ def main():
         "samples_per_s": args.res * args.res * args.spp * args.steps / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": EXTEND_TRAFFIC_BYTES,
-                     "kernel": "k_extend (nearest-hit traversal)",
+                     "kernel": "extend (nearest-hit traversal: k_extend_packet for camera rays, k_extend after)",
                      "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_launch_s * 1e3},
     }
     if args.calib_copy_mib:
@@ -437,9 +491,10 @@ def main():
         out["batch64"] = batch64()
     if not args.no_bpt:
         out["bpt"] = bpt_cornell(cpu=not args.no_cpu)
+    out["config0_cornell"] = cornell_config0(cpu=not args.no_cpu)
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(scene)
+        out["cpu_baseline"], out["cpu_baseline_1thread"] = cpu_baseline(scene)
     print(json.dumps(out), flush=True)
 
 
