@@ -134,6 +134,16 @@ def cornell_config0(spp=16, cpu=True):
     return out
 
 
+# The learned proposal's online warm-up as the guiding workload states it: 8
+# passes, each adding 2^18 radiance samples; Adam at lr 1e-2 (decaying to a
+# tenth over the warm-up), 64 steps of 16,384 samples after each pass drawn
+# from every sample the warm-up has kept so far.
+GUIDED_TRAINING = dict(train_passes=8, batch_size=16384, steps_per_pass=64, learning_rate=1e-2,
+                       samples_per_pass=1 << 18, flags=3)  # GUIDE_LR_DECAY | GUIDE_KEEP_SAMPLES
+GUIDED_TRAINING_DOC = ("8 warm-up passes x 2^18 new samples, kept across passes; 64 Adam steps of 16384 "
+                       "samples per pass over all kept samples; lr 1e-2 decaying to 1e-3")
+
+
 def equal_time_relmse(res=512, ref_spp=65536, budget_s=5.0, chunk=64):
     """The metric's second half, the learned-guiding workload as stated: the
     participating-media scene (Cornell box filled with a homogeneous medium,
@@ -141,9 +151,10 @@ def equal_time_relmse(res=512, ref_spp=65536, budget_s=5.0, chunk=64):
     depth 16.  The converged image is the plain GPU path tracer at 65,536 spp
     accumulated in FP64.  Plain, NEE-only and guided rendering each get the
     same wall-clock budget, training included; the guided renderer trains
-    online for 8 warm-up passes of 2^18 radiance samples (batch 16384 x 16
-    Adam steps, lr 1e-2), and a second guided line adds the learning-rate
-    decay option.  Each render call adds `chunk` samples per pixel; a renderer
+    online for 8 warm-up passes of 2^18 new radiance samples each
+    (GUIDED_TRAINING), and a second guided line keeps 16x that many samples
+    per pass (beyond the stated workload: what more data buys).  Each render
+    call adds `chunk` samples per pixel; a renderer
     runs until the budget has passed and its relMSE is scaled by elapsed /
     budget (Monte Carlo error falls as 1 / time), so renderers with longer
     calls gain nothing from the last call's overshoot."""
@@ -163,11 +174,13 @@ def equal_time_relmse(res=512, ref_spp=65536, budget_s=5.0, chunk=64):
     ref_s = time.perf_counter() - t
     out = {"scene": f"media+glass+Phong sphere {sc.n_tris} tris, {res}x{res} px, max depth 16",
            "reference": f"plain path tracing, {ref_spp} spp, FP64 accumulation ({ref_s:.0f} s)", "budget_s": budget_s,
-           "guided_training": "8 warm-up passes x 2^18 samples (batch 16384, 16 Adam steps per pass), lr 1e-2"}
+           "guided_training": GUIDED_TRAINING_DOC,
+           "guided_4m_training": "as guided, but each warm-up pass keeps 2^22 samples (16x the stated 2^18) "
+                                 "and trains on its own samples only (256 Adam steps, one epoch)"}
     runs = [("plain", api.MODE_PT, None), ("nee", api.MODE_NEE, None),
-            ("guided", api.MODE_GUIDED, dict(train_passes=8, batch_size=16384, steps_per_pass=16, learning_rate=1e-2)),
-            ("guided_lr_decay", api.MODE_GUIDED,
-             dict(train_passes=8, batch_size=16384, steps_per_pass=16, learning_rate=1e-2, flags=api.GUIDE_LR_DECAY))]
+            ("guided", api.MODE_GUIDED, GUIDED_TRAINING),
+            ("guided_4m", api.MODE_GUIDED,
+             dict(train_passes=8, batch_size=16384, steps_per_pass=256, learning_rate=1e-2, flags=api.GUIDE_LR_DECAY))]
     for name, mode, guide in runs:
         pt = api.PathTracerHIP(sc, ctx=ctx)
         if guide:
@@ -188,7 +201,7 @@ def equal_time_relmse(res=512, ref_spp=65536, budget_s=5.0, chunk=64):
         log(f"[bench] {name}: {n} spp in {el:.2f} s, relMSE {raw:.6f} ({out[name]:.6f} at {budget_s:.0f} s)")
     out["plain_over_guided"] = out["plain"] / out["guided"]
     out["nee_over_guided"] = out["nee"] / out["guided"]
-    out["nee_over_guided_lr_decay"] = out["nee"] / out["guided_lr_decay"]
+    out["nee_over_guided_4m"] = out["nee"] / out["guided_4m"]
     return out
 
 
@@ -198,8 +211,8 @@ def equal_time_glossy(res=512, ref_spp=65536, budget_s=3.0, chunk=64):
     converged image is NEE-only at 65,536 spp accumulated in FP64 (unbiased
     for the same mean as plain path tracing, and with far less noise than a
     plain render of that length).  Plain, NEE-only and guided rendering each
-    get the same wall-clock budget, training included (guided: 8 warm-up
-    passes of 2^18 samples, batch 16384 x 16 Adam steps, lr 1e-2)."""
+    get the same wall-clock budget, training included (guided: the warm-up
+    of GUIDED_TRAINING)."""
     from renderx_amd import api, scenes, film
 
     sc = scenes.icosphere_scene(res, res, max_depth=10)
@@ -215,10 +228,9 @@ def equal_time_glossy(res=512, ref_spp=65536, budget_s=3.0, chunk=64):
     ref = (acc / ref_spp).astype(np.float32)
     out = {"scene": f"Phong-glossy icosphere {sc.n_tris} tris, {res}x{res} px, max depth 10",
            "reference": f"NEE-only, {ref_spp} spp, FP64 accumulation ({time.perf_counter() - t:.0f} s)",
-           "budget_s": budget_s}
+           "budget_s": budget_s, "guided_training": GUIDED_TRAINING_DOC}
     for name, mode, guide in [("plain", api.MODE_PT, None), ("nee", api.MODE_NEE, None),
-                              ("guided", api.MODE_GUIDED,
-                               dict(train_passes=8, batch_size=16384, steps_per_pass=16, learning_rate=1e-2))]:
+                              ("guided", api.MODE_GUIDED, GUIDED_TRAINING)]:
         pt = api.PathTracerHIP(sc, ctx=ctx)
         if guide:
             pt.guide_init(seed=7, **guide)
@@ -247,7 +259,7 @@ GUIDE_TRAIN_FLOP = GUIDE_FWD_FLOP + 2 * (64 * 41 + 64 * 64 + 64 * 64) + GUIDE_FW
 
 def guided_profile(res=512, calls=4, chunk=64):
     """One guided render of the media + glass scene as the equal-time study
-    runs it (8 warm-up passes of 2^18 training samples, then `calls` render
+    runs it (the GUIDED_TRAINING warm-up, then `calls` render
     calls of `chunk` samples per pixel), and the same samples NEE-only.  The
     network rows inferred and trained give the MFMA kernels' FLOP counts
     (GUIDE_FWD_FLOP / GUIDE_TRAIN_FLOP per row) for a kernel trace of this
@@ -262,7 +274,7 @@ def guided_profile(res=512, calls=4, chunk=64):
     for name, mode in [("guided", api.MODE_GUIDED), ("nee", api.MODE_NEE)]:
         pt = api.PathTracerHIP(sc, ctx=ctx)
         if mode == api.MODE_GUIDED:
-            pt.guide_init(seed=7, train_passes=8, batch_size=16384, steps_per_pass=16, learning_rate=1e-2)
+            pt.guide_init(seed=7, **GUIDED_TRAINING)
         rows = trows = 0
         ms = []
         for k in range(calls):

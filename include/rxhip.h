@@ -337,21 +337,26 @@ int rx_scene_file_get(const rx_scene_file* file, float* verts, uint32_t* tris, r
 /* ------------------------------------------------------------------ path guiding */
 typedef struct rx_guide_params {
   uint32_t train_passes;   /* online warm-up passes: each renders enough samples per pixel for
-                              batch_size * steps_per_pass paths (at least one), the leading samples
+                              its store of radiance samples (at least one), the leading samples
                               of every pixel, and trains on them */
   uint32_t batch_size;     /* radiance samples per Adam step (>= 256) */
-  uint32_t steps_per_pass; /* Adam steps after each training pass, over deterministic
-                              permutations of that pass's samples (a store of at most
-                              batch_size * steps_per_pass samples, an even subset) */
+  uint32_t steps_per_pass; /* Adam steps after each training pass, over a deterministic
+                              permutation of that pass's store; steps beyond store / batch_size
+                              go over it again (epochs) */
   float learning_rate;     /* Adam */
   float beta1, beta2, eps;
   uint32_t seed;
   uint32_t flags;          /* rx_guide_flags */
+  uint32_t samples_per_pass; /* radiance samples a training pass keeps (the store: an even subset
+                              of its records); 0 = batch_size * steps_per_pass */
 } rx_guide_params;
 
 enum rx_guide_flags {
-  RX_GUIDE_LR_DECAY = 1    /* the Adam learning rate falls geometrically from learning_rate to a tenth of it
+  RX_GUIDE_LR_DECAY = 1,   /* the Adam learning rate falls geometrically from learning_rate to a tenth of it
                               over the online warm-up's train_passes * steps_per_pass steps */
+  RX_GUIDE_KEEP_SAMPLES = 2 /* each warm-up pass adds its samples_per_pass samples to those of the earlier
+                              passes, and its Adam steps draw from all of them (pass k of the warm-up
+                              trains on k * samples_per_pass samples) */
 };
 
 /* The guiding network has no counterpart in the reference.  It maps a

@@ -29,7 +29,8 @@ RENDER_TIMING, RENDER_COUNT_WORK = 1, 2
 RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use_graph", "<u4"), ("stack_depth", "<u4"),
                           ("block_size", "<u4"), ("camera_rays", "<u4")])
 GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("batch_size", "<u4"), ("steps_per_pass", "<u4"), ("learning_rate", "<f4"),
-                         ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4")])
+                         ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4"),
+                         ("samples_per_pass", "<u4")])
 MODE_PT, MODE_NEE, MODE_GUIDED, MODE_BPT = 0, 1, 2, 3
 
 
@@ -39,6 +40,7 @@ def _tuning(path_slots, blocks_per_cu, use_graph, stack_depth, block_size, camer
     t["stack_depth"], t["block_size"], t["camera_rays"] = stack_depth, block_size, camera_rays
     return t
 GUIDE_LR_DECAY = 1
+GUIDE_KEEP_SAMPLES = 2
 
 
 class KDTree:
@@ -148,10 +150,14 @@ class PathTracerHIP:
         self.ctx.check(lib().rx_load_scene(self.ctx.h, len(v), ptr(v), len(t), ptr(t), len(o), ptr(o),
                                            1 if tree_abs_int else 0), "rx_load_scene")
 
-    def guide_init(self, train_passes=16, batch_size=4096, steps_per_pass=16, learning_rate=1e-2, seed=7, flags=0):
-        """rx_guide_init: the network and its online warm-up (flags: GUIDE_LR_DECAY)."""
+    def guide_init(self, train_passes=16, batch_size=4096, steps_per_pass=16, learning_rate=1e-2, seed=7, flags=0,
+                   samples_per_pass=0):
+        """rx_guide_init: the network and its online warm-up (flags: GUIDE_LR_DECAY).  samples_per_pass:
+        radiance samples each training pass keeps (0: batch_size * steps_per_pass); more steps than
+        samples_per_pass / batch_size go over them again."""
         p = np.zeros(1, GUIDE_PARAMS)
         p["flags"] = flags
+        p["samples_per_pass"] = samples_per_pass
         p["train_passes"] = train_passes
         p["batch_size"] = batch_size
         p["steps_per_pass"] = steps_per_pass
@@ -159,6 +165,12 @@ class PathTracerHIP:
         p["beta1"], p["beta2"], p["eps"] = 0.9, 0.999, 1e-8
         p["seed"] = seed
         self.ctx.check(lib().rx_guide_init(self.ctx.h, ptr(p)), "rx_guide_init")
+
+    def guide_weights(self):
+        """rx_guide_get_weights: the network's FP32 master parameters."""
+        w = np.zeros(_lib.GUIDE_PARAMS, np.float32)
+        self.ctx.check(lib().rx_guide_get_weights(self.ctx.h, ptr(w)), "rx_guide_get_weights")
+        return w
 
     def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0, stack_depth=0, block_size=0, camera_rays=0):
         """Launch shape of renderPixels (rx_render_tuning); images do not depend on it."""

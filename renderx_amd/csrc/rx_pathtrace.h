@@ -1516,9 +1516,11 @@ static __global__ void k_rec_write(const float* rec, uint32_t K, float tag, cons
 // sample ((step * batch + r) * 0x7fffffff) mod n, a permutation of the n
 // stored samples (the prime multiplier is coprime to any n < 2^31), so
 // consecutive steps sweep the store in a scattered, deterministic order.
-static __global__ void k_guide_gather(TrainOut S, const uint32_t* total, uint32_t store_cap, uint32_t step, uint32_t batch,
-                               TrainOut D) {
-  const uint32_t n = min(*total, store_cap);
+// n is the pass's own samples, min(*total, store_cap), or with kept samples
+// (kept > 0) the kept store's first `kept` rows.
+static __global__ void k_guide_gather(TrainOut S, const uint32_t* total, uint32_t store_cap, uint32_t kept, uint32_t step,
+                                      uint32_t batch, TrainOut D) {
+  const uint32_t n = kept ? kept : min(*total, store_cap);
   const uint32_t rows = min(n, batch);
   if (blockIdx.x == 0 && threadIdx.x == 0) *D.rows = rows;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {

@@ -258,7 +258,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   const int mode = (int)prm->mode;
   GuideArgs GA{};
   const BatchArgs BA0{};
-  uint32_t train_passes = 0, batch = 0, steps = 0, store_cap = 0, s_train = 1, train_end = 0;
+  uint32_t train_passes = 0, batch = 0, steps = 0, store_cap = 0, store_total = 0, s_train = 1, train_end = 0;
+  bool keep = false;
   if (mode == RX_MODE_GUIDED) {
     if (!c->guide) return set_err(c, RX_ERR_STATE, "rx_render: guided mode needs rx_guide_init");
     GuideState& G = *c->guide;
@@ -269,15 +270,22 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     }
     // The online warm-up: train_passes passes of s_train samples per pixel
     // each, the leading train_end samples of every pixel.  A pass has enough
-    // paths for its sample store (batch * steps rows, an evenly strided
-    // subset of the pass's records) and at least kTrainPassPaths paths: a
+    // paths for its sample store (samples_per_pass rows, by default batch *
+    // steps, an evenly strided subset of the pass's records) and at least
+    // kTrainPassPaths paths: a
     // pass with fewer waits on its longest paths at every bounce.  A render
     // resumed with sample_offset >= train_end only uses the network.
     constexpr uint32_t kTrainPassPaths = 1u << 20;
     train_passes = G.prm.train_passes;
     batch = std::max(256u, G.prm.batch_size);
     steps = std::max(1u, G.prm.steps_per_pass);
-    store_cap = (uint32_t)std::min<uint64_t>((uint64_t)batch * steps, 1u << 26);
+    const uint64_t want = G.prm.samples_per_pass ? G.prm.samples_per_pass : (uint64_t)batch * steps;
+    store_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, batch), 1u << 26);
+    // with kept samples the store holds every warm-up pass's samples, pass k at rows k * store_cap
+    keep = (G.prm.flags & RX_GUIDE_KEEP_SAMPLES) != 0;
+    if (keep && (uint64_t)store_cap * std::max(1u, G.prm.train_passes) > (1u << 27))
+      return set_err(c, RX_ERR_INVALID, "rx_render: kept guide samples (samples_per_pass * train_passes) exceed 2^27");
+    store_total = keep ? store_cap * std::max(1u, G.prm.train_passes) : store_cap;
     s_train = std::max((store_cap + npix - 1) / npix, (kTrainPassPaths + npix - 1) / npix);
     s_train = std::max(1u, std::min(S_max, s_train));
     train_end = train_passes * s_train;
@@ -288,7 +296,11 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       return set_err(c, RX_ERR_INVALID, "rx_render: guided mode needs pixels * (max_depth + 1) < 2^31");
     if (int r = guide_reserve(c, G, batch)) return r;  // the training step's rows
     if (prm->sample_offset < train_end && prm->spp) {
-      if (int r = guide_reserve_store(c, G, store_cap)) return r;
+      if (int r = guide_reserve_store(c, G, (store_total + 1) & ~1u)) return r;
+      // the warm-up starts (or the network was re-initialised): no kept samples yet (zero weights add
+      // nothing to the loss)
+      if (keep && (prm->sample_offset == 0 || G.step == 0))
+        hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, s, G.swt.as<double>(), (size_t)(store_total + 1) / 2);
       const size_t K = (size_t)npix * s_train * Rs;
       RXCK(R.rec.ensure(K * kRecFloats * 4), "alloc records");
       RXCK(R.bcount.ensure((K / kRecChunk + 2) * 4), "alloc records");
@@ -389,7 +401,13 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(256), 0, s, GA.rec, K, GA.tag, R.bcount.as<uint32_t>());
       hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(kScanThreads), 0, s, R.bcount.as<uint32_t>(), nb,
                          R.total.as<uint32_t>());
-      const TrainOut ST{G.spos.as<float4>(), G.sdir.as<float4>(), G.swdir.as<float4>(), G.spb.as<float>(),
+      // this warm-up pass's rows of the store (with kept samples, after the earlier passes' rows)
+      const uint32_t tp = std::min(P.sample / s_train, std::max(1u, train_passes) - 1);
+      const size_t off = keep ? (size_t)tp * store_cap : 0;
+      const uint32_t kept = keep ? (tp + 1) * store_cap : 0;
+      const TrainOut ST{G.spos.as<float4>() + off, G.sdir.as<float4>() + off, G.swdir.as<float4>() + off,
+                        G.spb.as<float>() + off, G.swt.as<float>() + off, G.srows.as<uint32_t>()};
+      const TrainOut SK{G.spos.as<float4>(), G.sdir.as<float4>(), G.swdir.as<float4>(), G.spb.as<float>(),
                         G.swt.as<float>(), G.srows.as<uint32_t>()};
       const TrainOut TB{G.pos.as<float4>(), G.dir.as<float4>(), G.wdir.as<float4>(), G.pb.as<float>(),
                         G.wt.as<float>(), G.rows.as<uint32_t>()};
@@ -397,8 +415,8 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
                          R.total.as<uint32_t>(), store_cap, B.L, GA.R, ST);
       RXCK(hipGetLastError(), "launch record write");
       for (uint32_t k = 0; k < steps; k++) {
-        hipLaunchKernelGGL(k_guide_gather, dim3(std::min<uint32_t>((batch + 255) / 256, 1024u)), dim3(256), 0, s, ST,
-                           R.total.as<uint32_t>(), store_cap, k, batch, TB);
+        hipLaunchKernelGGL(k_guide_gather, dim3(std::min<uint32_t>((batch + 255) / 256, 1024u)), dim3(256), 0, s, SK,
+                           R.total.as<uint32_t>(), store_cap, kept, k, batch, TB);
         RXCK(hipGetLastError(), "launch gather");
         if (int r = guide_train_rows(c, batch)) return r;
         train_rows += batch;

@@ -30,7 +30,7 @@ _Static_assert(sizeof(rx_render_tuning) == 24 && offsetof(rx_render_tuning, stac
 _Static_assert(sizeof(rx_render_stats) == 112 && offsetof(rx_render_stats, ms_total) == 56 &&
                    offsetof(rx_render_stats, extend_launches) == 88 && offsetof(rx_render_stats, guide_rows) == 96,
                "rx_render_stats layout");
-_Static_assert(sizeof(rx_guide_params) == 36, "rx_guide_params layout");
+_Static_assert(sizeof(rx_guide_params) == 40, "rx_guide_params layout");
 
 static int fails = 0;
 #define CHECK(cond, what)                                   \

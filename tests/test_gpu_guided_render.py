@@ -4,7 +4,8 @@ estimator); at equal render time (training included) it beats both the plain
 GPU path tracer (by at least 2x) and NEE without guiding in relMSE against a
 converged image on the media + glass scene, beats the plain path tracer by at
 least 2x on the glossy config, and the trained network beats the same
-estimator with the untrained network."""
+estimator with the untrained network; warm-up passes that keep their samples
+train reproducibly and stay unbiased."""
 import time
 
 import numpy as np
@@ -105,9 +106,9 @@ def test_equal_time_relmse_guided_vs_nee_and_plain(rx, gpu_ctx):
     res = {}
     for name, mode in [("plain", MODE_PT), ("nee", MODE_NEE), ("guided", MODE_GUIDED)]:
         pt = rx.PathTracerHIP(sc, ctx=gpu_ctx)
-        if mode == MODE_GUIDED:
-            pt.guide_init(train_passes=16, batch_size=16384, steps_per_pass=16, learning_rate=1e-2, seed=7,
-                          flags=rx.GUIDE_LR_DECAY)
+        if mode == MODE_GUIDED:  # the guiding workload's warm-up: 8 passes of 2^18 new samples, kept
+            pt.guide_init(train_passes=8, batch_size=16384, steps_per_pass=64, learning_rate=1e-2, seed=7,
+                          samples_per_pass=1 << 18, flags=rx.GUIDE_LR_DECAY | rx.GUIDE_KEEP_SAMPLES)
         img, n, dt = _equal_time(pt, mode, budget, 64, seed=5)
         res[name] = rel_mse(img, ref) * dt / budget
         print(f"  {name:7s}: {n:6d} spp in {dt:.2f} s, relMSE {res[name]:.6f}")
@@ -137,6 +138,34 @@ def test_equal_time_relmse_glossy(rx, gpu_ctx):
         print(f"  {name:7s}: {n:6d} spp in {dt:.2f} s, relMSE {res[name]:.7f}")
     print(f"  guided vs plain: {res['plain'] / res['guided']:.2f}x, guided vs NEE-only: {res['nee'] / res['guided']:.2f}x")
     assert res["guided"] * 2 <= res["plain"]
+
+
+def test_kept_samples_warm_up(rx, gpu_ctx):
+    """Warm-up passes that keep their samples (RX_GUIDE_KEEP_SAMPLES): pass k
+    of the warm-up trains on the samples of passes 1..k.  The Adam step count
+    is the one asked for, the network differs from training on each pass's own
+    samples only, renders are bit-reproducible, and the image keeps the plain
+    path tracer's mean."""
+    print("[guided] kept warm-up samples", flush=True)
+    sc = scenes.media_scene(24, 24, max_depth=16, freq=24, glass_freq=8)
+    ref = rx.PathTracerHIP(sc, ctx=gpu_ctx).renderPixels(spp=1 << 18, mode=MODE_PT, seed=77)
+    cfg = dict(train_passes=4, batch_size=1024, steps_per_pass=8, learning_rate=1e-2, seed=7, samples_per_pass=4096)
+    gn = 1 << 16
+    runs, weights = [], []
+    for flags in (rx.GUIDE_KEEP_SAMPLES, rx.GUIDE_KEEP_SAMPLES, 0):
+        pt = rx.PathTracerHIP(sc, ctx=gpu_ctx)
+        pt.guide_init(flags=flags, **cfg)
+        runs.append(pt.renderPixels(spp=gn, mode=MODE_GUIDED, seed=21))
+        weights.append(pt.guide_weights())
+    r = runs[0]
+    assert int(r["stats"]["guide_train_rows"]) == 4 * 8 * 1024
+    assert np.array_equal(r["sum"], runs[1]["sum"]) and np.array_equal(weights[0], weights[1])
+    assert not np.array_equal(weights[0], weights[2])
+    mask = clt_mask(r["sum"], r["sumsq"], ref["sum"], ref["sumsq"])
+    frac, zmean, _ = three_sigma(r["sum"], r["sumsq"], gn, ref["sum"], ref["sumsq"], 1 << 18, pooled=False, mask=mask)
+    print(f"  within 3 sigma {frac:.4f}, mean z {zmean:+.4f}, excluded {1 - mask.mean():.4f}")
+    assert np.isfinite(r["sum"]).all() and mask.mean() >= 0.95
+    assert frac >= 0.993 and abs(zmean) < 0.1
 
 
 def test_trained_network_beats_untrained(rx, gpu_ctx):
