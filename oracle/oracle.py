@@ -55,6 +55,7 @@ def lib():
         L.orc_max_threads.restype = C.c_int
         L.orc_sampler_draws.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, P]
         L.orc_set_mutation.argtypes = [C.c_int]
+        L.orc_bpt_density_stats.argtypes = [P, P, P]
         L.orc_refract_dir.restype = C.c_int
         L.orc_refract_dir.argtypes = [C.c_int, P, P, C.c_float, C.c_float, P, P]
         L.orc_medium_densities.argtypes = [P, C.c_float, P, P]
@@ -162,12 +163,13 @@ class SceneOracle:
         return Tree(self._v, self._t, _handle=lib().orc_scene_tree(self.h), _owner=self)
 
     def render(self, spp, max_depth=None, rr_start_depth=None, mode=0, seed=1, sample_offset=0, threads=0,
-               stats=None, registered=False):
+               stats=None, registered=False, density_check=False):
         """FP64 per-pixel (sum, sum of squares), shape (H, W, 3).  mode 0 is the
         PathTracer, 1 PathTracerTest, 3 the BidirectionalPathTracer; `registered` (mode 3) lands light-tracing contributions on
         the pixel that covers them instead of the reference's half-pixel-shifted
         one.  If `stats` is a dict it receives the number of traversal queries
-        under "rays"."""
+        under "rays"; with `density_check` (mode 3) also the balance weights'
+        density check (bpt_density_stats)."""
         cam = self.scene.camera
         H, W = int(cam["height"]), int(cam["width"])
         p = np.zeros((), RENDER_PARAMS)
@@ -177,7 +179,7 @@ class SceneOracle:
         p["mode"] = mode
         p["seed"] = seed
         p["sample_offset"] = sample_offset
-        p["flags"] = 1 if registered else 0
+        p["flags"] = (1 if registered else 0) | (2 if density_check else 0)
         p = p.reshape(1)
         s = np.zeros((H, W, 3), np.float64)
         q = np.zeros((H, W, 3), np.float64)
@@ -185,7 +187,20 @@ class SceneOracle:
         lib().orc_render(self.h, _p(p), _p(s), _p(q), threads, _p(rays))
         if stats is not None:
             stats["rays"] = int(rays[0])
+            if density_check:
+                stats.update(bpt_density_stats())
         return s, q
+
+
+def bpt_density_stats():
+    """The last density-checked bidirectional render: connections compared,
+    those among them with a DEFINITE vertex ("skipped": that step's densities
+    are left out on both sides), and the largest relative
+    difference between the densities of the balance weight's sampled
+    prefixes and the densities the subpaths were drawn with."""
+    c, k, m = C.c_uint64(), C.c_uint64(), C.c_double()
+    lib().orc_bpt_density_stats(C.byref(c), C.byref(k), C.byref(m))
+    return {"checked": c.value, "skipped": k.value, "max_rel": m.value}
 
 
 def mt_nth(n: int) -> int:
@@ -201,7 +216,8 @@ def sampler_draws(kind: int, n: int, param: float, param2: float = 0.0, seed: in
 
 # ---------------------------------------------------------------- single functions, for their pins
 # Planted mistakes (see rx_oracle_scene.h): each must make one pin fail.
-MUT_NONE, MUT_ROTATION_SIGN, MUT_INVERTED_RATIO, MUT_P_SURFACE_COMPLEMENT, MUT_P_MEDIUM_NO_SIGMA, MUT_GLOSSY_AXIS = range(6)
+(MUT_NONE, MUT_ROTATION_SIGN, MUT_INVERTED_RATIO, MUT_P_SURFACE_COMPLEMENT, MUT_P_MEDIUM_NO_SIGMA, MUT_GLOSSY_AXIS,
+ MUT_ORIGIN_PDF_SWAP) = range(7)
 
 
 class mutation:

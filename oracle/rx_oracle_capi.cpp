@@ -250,6 +250,9 @@ void orc_sampler_draws(int kind, int n, float param, float param2, uint64_t seed
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
 void orc_set_mutation(int m) { g_mutation = m; }
+void orc_bpt_density_stats(uint64_t* checked, uint64_t* skipped, double* max_rel) {
+  bpt_density_stats(checked, skipped, max_rel);
+}
 int orc_refract_dir(int kind, const float* d, const float* n, float n_i, float n_t, float* dir, float* er) {
   return pin_refract(kind, d, n, n_i, n_t, dir, er) ? 1 : 0;
 }

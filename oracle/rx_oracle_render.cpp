@@ -15,6 +15,7 @@
 //  * the wasted intersection inside SceneObject::emit (src/SceneObject.cpp:198)
 //    is skipped: its result is never read by PathTracerTest.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -883,7 +884,20 @@ static float origin_pdf(const SceneO& sc, const PRay& in, const PRay& out) {
   if (in.insideObject < 0 || sc.objs[in.insideObject].type != OBJ_VPM) return 1.f;
   const float dist = std::max(length(in.origin - out.origin), EPS);
   const ObjO& o = sc.objs[in.insideObject];
-  return out.contactObject != NONE ? P_surface(o, dist) : p_medium(o, dist);
+  const bool surface = (out.contactObject != NONE) != (g_mutation == MUT_ORIGIN_PDF_SWAP);
+  return surface ? P_surface(o, dist) : p_medium(o, dist);
+}
+
+// Density check of the balance weights (test hook, RenderParamsO::flags bit 1).
+static bool g_density_check = false;
+static std::atomic<uint64_t> g_dc_checked{0}, g_dc_skipped{0};
+static std::atomic<uint64_t> g_dc_max_bits{0};  // bits of the largest relative difference (a non-negative double)
+static void dc_record(double rel) {
+  uint64_t bits;
+  std::memcpy(&bits, &rel, 8);
+  uint64_t cur = g_dc_max_bits.load();
+  while (bits > cur && !g_dc_max_bits.compare_exchange_weak(cur, bits)) {
+  }
 }
 
 // BidirectionalPathTracer::mustUsePT (:124-131)
@@ -983,6 +997,52 @@ static float connect_weight(const SceneO& sc, const Path& path, int ci, bool use
       pb[i - 1] *= origin_pdf(sc, bpt_link(path, ci, i, i - 1), bpt_link(path, ci, i - 1, i - 2));
     }
   }
+  if (g_density_check && ci >= 0 && ci + 1 < n) {
+    // The light subpath drew vertices 0..ci, the eye subpath n-1..ci+1.  Each
+    // density factor connectWeight uses for those sampled steps is compared
+    // with the one the step was drawn with: the origin density of the vertex
+    // a step leads to (the ray's originProb: P_surface or p_medium of the
+    // medium crossed, 1 outside media), and the direction density at a
+    // vertex (the ray's directionProb) where the reference's two agree:
+    // emission, the camera, diffuse surfaces and scattering points in a
+    // medium.  (At a glossy or Phong vertex getDirectionSampleProbDensity
+    // leaves out the roulette factor, or sums both lobes, that the scatter
+    // includes; at a DEFINITE vertex connectWeight includes neither factor.)
+    auto diffuse_or_medium = [&](const PRay& r) {
+      if (r.contactObject == CAMERA) return true;
+      if (r.contactObject >= 0) return sc.objs[r.contactObject].type == OBJ_DIFFUSE;
+      return r.insideObject >= 0 && sc.objs[r.insideObject].type == OBJ_VPM;
+    };
+    double worst = 0;
+    auto cmp = [&](double model, double drawn) {
+      const double rel = std::fabs(model / drawn - 1.0);
+      worst = std::max(worst, std::isfinite(rel) ? rel : 1e300);
+    };
+    bool definite = false;
+    for (int i = 0; i < ci; i++) {
+      if (path[i].directionSampleType != RANDOM) {
+        definite = true;
+        continue;
+      }
+      cmp(origin_pdf(sc, bpt_link(path, ci, i, i + 1), bpt_link(path, ci, i + 1, i + 2)), path[i + 1].originProb);
+      if (i == 0) cmp(self_dir_pdf(sc, bpt_link(path, ci, 0, 1)), path[0].directionProb);
+      else if (diffuse_or_medium(path[i]))
+        cmp(dir_pdf(sc, bpt_link(path, ci, i - 1, i), bpt_link(path, ci, i, i + 1)), path[i].directionProb);
+    }
+    for (int i = n - 1; i > ci + 1; i--) {
+      if (path[i].directionSampleType != RANDOM) {
+        definite = true;
+        continue;
+      }
+      cmp(origin_pdf(sc, bpt_link(path, ci, i, i - 1), bpt_link(path, ci, i - 1, i - 2)), path[i - 1].originProb);
+      if (i == n - 1) cmp(self_dir_pdf(sc, bpt_link(path, ci, i, i - 1)), path[i].directionProb);
+      else if (diffuse_or_medium(path[i]))
+        cmp(dir_pdf(sc, bpt_link(path, ci, i + 1, i), bpt_link(path, ci, i, i - 1)), path[i].directionProb);
+    }
+    dc_record(worst);
+    g_dc_checked++;
+    if (definite) g_dc_skipped++;
+  }
   double sum = 0;
   for (int i = 0; i < n - 1; i++)
     if (path[i].directionSampleType == RANDOM && path[i + 1].directionSampleType == RANDOM) sum += pf[i] * pb[i + 1];
@@ -1077,6 +1137,12 @@ static void connect_paths(Ctx& c, unsigned pixel, const Path& eyePath, const Pat
 // one light subpath (src/BidirectionalPathTracer.cpp:36-57), the iteration
 // image divided by the vignetting factor (:100) and added to the FP64 sums.
 static void render_bpt(const SceneO& sc, const RenderParamsO& p, double* sum, double* sumsq, uint64_t* rays) {
+  g_density_check = (p.flags & 2u) != 0;
+  if (g_density_check) {
+    g_dc_checked = 0;
+    g_dc_skipped = 0;
+    g_dc_max_bits = 0;
+  }
   const unsigned npix = (unsigned)(sc.cam.width * sc.cam.height);
   std::vector<double> img(3 * (size_t)npix);
   uint64_t total_rays = 0;
@@ -1217,6 +1283,13 @@ uint32_t mt_nth(uint32_t n) {
   uint32_t v = 0;
   for (uint32_t i = 0; i < n; i++) v = m.next();
   return v;
+}
+
+void bpt_density_stats(uint64_t* checked, uint64_t* skipped, double* max_rel) {
+  *checked = g_dc_checked.load();
+  *skipped = g_dc_skipped.load();
+  const uint64_t bits = g_dc_max_bits.load();
+  std::memcpy(max_rel, &bits, 8);
 }
 
 }  // namespace orc

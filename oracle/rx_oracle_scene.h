@@ -71,9 +71,18 @@ enum Mutation {
   MUT_INVERTED_RATIO = 2,        // n_t / n_i instead of n_i / n_t
   MUT_P_SURFACE_COMPLEMENT = 3,  // P_surface = y(1 - e^{-sigma_t d})
   MUT_P_MEDIUM_NO_SIGMA = 4,     // p_medium without its sigma_t factor
-  MUT_GLOSSY_AXIS = 5            // glossy lobe about -d instead of the mirror direction
+  MUT_GLOSSY_AXIS = 5,           // glossy lobe about -d instead of the mirror direction
+  MUT_ORIGIN_PDF_SWAP = 6        // the balance weights' origin density: P_surface and p_medium swapped
 };
 extern int g_mutation;
+// Bidirectional renders with RenderParamsO::flags bit 1: every connection
+// compares the densities its balance weight uses for the sampled prefixes
+// (the products pf / pb of BidirectionalPathTracer::connectWeight) with the
+// densities the subpaths were actually drawn with (each ray's directionProb
+// and originProb), under the same geometry factors.  Counts (connections
+// compared; those among them with a DEFINITE vertex, whose step neither side
+// includes) and the largest relative difference since the last reset.
+void bpt_density_stats(uint64_t* checked, uint64_t* skipped, double* max_rel);
 // The oracle's own functions, for the tests that pin them:
 // kind 0 = RefractiveMaterial's step, 1 = SceneVPMObject's; false on TIR.
 bool pin_refract(int kind, const float* d, const float* n, float n_i, float n_t, float* dir, float* er);

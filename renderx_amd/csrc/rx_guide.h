@@ -84,13 +84,16 @@ __device__ __forceinline__ Vec3 guide_sample(const GuideHead& h, float u0, float
   return normalize(m * w + t1 * (r * cp) + t2 * (r * sp));
 }
 
-// Frequency encoding of one query into 96 bf16 values.
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// FP32 -> BF16, round to nearest even: gfx950's v_cvt_pk_bf16_f32, two
+// values per instruction for bf2_pack.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// (bf16(lo) | bf16(hi) << 16)
+__device__ __forceinline__ uint32_t bf2_pack(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){lo, hi}, bf16x2_t));
 }
+// Frequency encoding of one query into 96 bf16 values.
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // Layout of the 96 values: (sin, cos) pairs, octave-major: position octave k
@@ -106,14 +109,14 @@ __device__ __forceinline__ void guide_encode(const GuideNet& N, Vec3 p, Vec3 d, 
     for (int k = 0; k < 12; k++) {
       float sn, cs;
       sincospif(f, &sn, &cs);
-      w[3 * k + a] = (uint32_t)f2bf(sn) | ((uint32_t)f2bf(cs) << 16);
+      w[3 * k + a] = bf2_pack(sn, cs);
       f *= 2.0f;
     }
     f = dc[a];
     for (int k = 0; k < 4; k++) {
       float sn, cs;
       sincospif(f, &sn, &cs);
-      w[36 + 3 * k + a] = (uint32_t)f2bf(sn) | ((uint32_t)f2bf(cs) << 16);
+      w[36 + 3 * k + a] = bf2_pack(sn, cs);
       f *= 2.0f;
     }
   }
@@ -138,7 +141,7 @@ __device__ __forceinline__ uint32_t guide_pair(const GuideNet& N, Vec3 p, Vec3 d
   }
   float sn, cs;
   sincospif(f, &sn, &cs);
-  return (uint32_t)f2bf(sn) | ((uint32_t)f2bf(cs) << 16);
+  return bf2_pack(sn, cs);
 }
 
 // ------------------------------------------------------------------ MFMA tiles

@@ -49,10 +49,9 @@ __device__ __forceinline__ void fwd_layer(const uint16_t* w, int ldw, const floa
   }
 }
 __device__ __forceinline__ v4s relu_bf16(v4f a) {
-  v4s r;
-#pragma unroll
-  for (int i = 0; i < 4; i++) r[i] = (short)f2bf(fmaxf(a[i], 0.0f));
-  return r;
+  const uint32_t p0 = bf2_pack(fmaxf(a[0], 0.0f), fmaxf(a[1], 0.0f));
+  const uint32_t p1 = bf2_pack(fmaxf(a[2], 0.0f), fmaxf(a[3], 0.0f));
+  return v4s{(short)(p0 & 0xffffu), (short)(p0 >> 16), (short)(p1 & 0xffffu), (short)(p1 >> 16)};
 }
 // LDS written by some lanes of a wave, read by others: the wave's LDS
 // operations complete in order once issued.
@@ -98,7 +97,7 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
           sn = s2;
           cs = c2;
         }
-        pr[u] = (uint32_t)f2bf(sn) | ((uint32_t)f2bf(cs) << 16);
+        pr[u] = bf2_pack(sn, cs);
       }
       in[ks] = v4s{(short)(pr[0] & 0xffffu), (short)(pr[0] >> 16), (short)(pr[1] & 0xffffu), (short)(pr[1] >> 16)};
     }
@@ -145,7 +144,9 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
       const float total = pair + p1 + p2 + p3;
       // lanes sub = 0..3 hold pairs 0..3; the sum of the pairs before this lane's
       const float before = (sub == 0) ? 0.0f : (sub == 1) ? p1 : (sub == 2) ? (p2 + p3) : (p1 + p2 + p3);
-      const float inv = 1.0f / total;
+      // approximate reciprocals (1 ulp) in the decode: the sampler and the
+      // density both read the stored values, so they stay consistent
+      const float inv = __builtin_amdgcn_rcpf(total);
       const float cdf0 = (before + e0) * inv;
       const float cdf1 = sub == 3 ? 1.0f : (before + e0 + e1) * inv;
       const float cprev = before * inv;
@@ -155,11 +156,12 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
       for (int u = 0; u < 2; u++) {
         const int j = u ? j1 : j0;
         const float m0 = zr[5 * j + 1], m1 = zr[5 * j + 2], m2 = zr[5 * j + 3];
-        const float l = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
+        const float il = __builtin_amdgcn_rsqf(fmaxf(m0 * m0 + m1 * m1 + m2 * m2, 1e-12f));
         const float k = __expf(fminf(8.0f, fmaxf(-3.0f, zr[5 * j + 4])));
         if (out) {
-          reinterpret_cast<float4*>(hrow)[j] = make_float4(m0 / l, m1 / l, m2 / l, k);
-          reinterpret_cast<float*>(hrow + kGHeadCw)[j] = wts2[u] * (k / (6.28318530717958647692f * (1.0f - __expf(-2.0f * k))));
+          reinterpret_cast<float4*>(hrow)[j] = make_float4(m0 * il, m1 * il, m2 * il, k);
+          reinterpret_cast<float*>(hrow + kGHeadCw)[j] =
+              wts2[u] * k * __builtin_amdgcn_rcpf(6.28318530717958647692f * (1.0f - __expf(-2.0f * k)));
           reinterpret_cast<float*>(hrow + kGHeadCdf)[j] = cdfs[u];
         }
       }

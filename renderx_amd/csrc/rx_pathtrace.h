@@ -715,9 +715,14 @@ __device__ __forceinline__ void bsdf_nrr(const DevObject& O, Vec3 n, Vec3 d, Vec
     pdf = pd + pg;
   }
 }
+// Russian roulette at a guided vertex: the continuation probability of the
+// reference's scatter (src/DiffuseMaterial.cpp:49-70, GlossyMaterial.cpp:31-53);
+// for a Phong object its mean over the uniform lobe pick, (max ks + max kd) / 2,
+// so guided paths are no longer on average than the plain estimator's (the
+// lobe sum max(ks + kd), twice that, made guided paths on the glossy config
+// 45 % more expensive to trace for little less variance).
 __device__ __forceinline__ float guided_rr(const DevObject& O) {
-  if (O.type == OBJ_PHONG)
-    return fminf(1.0f, fmaxf(O.c1[0] + O.c2[0], fmaxf(O.c1[1] + O.c2[1], O.c1[2] + O.c2[2])));
+  if (O.type == OBJ_PHONG) return fminf(1.0f, 0.5f * (O.max1 + O.max2));
   return O.max1;
 }
 

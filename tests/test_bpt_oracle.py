@@ -62,8 +62,8 @@ def test_registered_bpt_in_fog_matches_pathtracer(oracle_mod):
     (PathTracerTest's next-event connections carry no transmittance in a
     medium, so the plain PathTracer is the reference here).  The image mean
     cannot see the distance densities in the balance weights (weights formed
-    from the same densities sum to one whatever they are); those densities are
-    the medium densities that test_oracle_pins pins."""
+    from the same densities sum to one whatever they are); the next test pins
+    them against the densities the subpaths are drawn with."""
     sc = scenes.fog_room(12, 12)
     o = oracle_mod.SceneOracle(sc)
     nb, npt = 1024, 16384
@@ -78,3 +78,32 @@ def test_registered_bpt_in_fog_matches_pathtracer(oracle_mod):
     assert mask.mean() >= 0.95
     assert frac >= 0.99 and abs(zmean) < 0.15
     assert abs(mb - mp) < 0.02 * mp
+
+
+@pytest.mark.parametrize("name", ["fog", "media"])
+def test_balance_weight_densities_match_sampling(oracle_mod, name):
+    """BidirectionalPathTracer::connectWeight (src/BidirectionalPathTracer.cpp:197-263)
+    rebuilds the density of every strategy from getOriginSampleProbDensity
+    (src/Ray.cpp:42-50, src/SceneVPMObject.cpp:266-273) and
+    getDirectionSampleProbDensity.  For the steps the two subpaths actually
+    sampled, each such factor must equal the density the step was drawn with
+    (the ray's originProb, directionProb): the origin density of every vertex
+    (P_surface for a surface reached through a medium, p_medium for a
+    scattering point, 1 outside media), and the direction density at the
+    light, the camera, diffuse vertices and scattering points.  Checked on
+    every connection of a render in the fog room (camera and light inside the
+    medium) and in the media scene (glass, Phong, a medium entered through
+    its boundary); the planted mistake of P_surface and p_medium swapped in
+    the weights fails it."""
+    sc = scenes.fog_room(12, 12) if name == "fog" else scenes.media_scene(12, 12, max_depth=8, freq=12, glass_freq=6)
+    o = oracle_mod.SceneOracle(sc)
+    st = {}
+    o.render(64, mode=3, seed=3, stats=st, density_check=True)
+    print(f"  {name}: {st}")
+    assert st["checked"] >= (50000 if name == "fog" else 3000)
+    assert st["max_rel"] < 1e-4
+    with oracle_mod.mutation(oracle_mod.MUT_ORIGIN_PDF_SWAP):
+        bad = {}
+        o.render(64, mode=3, seed=3, stats=bad, density_check=True)
+    print(f"  {name}, P_surface and p_medium swapped in the weights: {bad}")
+    assert bad["max_rel"] > 0.1
