@@ -257,18 +257,23 @@ GUIDE_FWD_FLOP = 2 * (96 * 64 + 64 * 64 + 64 * 64 + 64 * 41)
 GUIDE_TRAIN_FLOP = GUIDE_FWD_FLOP + 2 * (64 * 41 + 64 * 64 + 64 * 64) + GUIDE_FWD_FLOP
 
 
-def guided_profile(res=512, calls=4, chunk=64):
-    """One guided render of the media + glass scene as the equal-time study
-    runs it (the GUIDED_TRAINING warm-up, then `calls` render
-    calls of `chunk` samples per pixel), and the same samples NEE-only.  The
-    network rows inferred and trained give the MFMA kernels' FLOP counts
-    (GUIDE_FWD_FLOP / GUIDE_TRAIN_FLOP per row) for a kernel trace of this
-    run."""
+def guided_profile(scene="media", res=512, calls=4, chunk=64):
+    """One guided render of the media + glass scene (or of the glossy
+    icosphere scene) as the equal-time study runs it (the GUIDED_TRAINING
+    warm-up, then `calls` render calls of `chunk` samples per pixel), and the
+    same samples NEE-only.  The network rows inferred and trained give the
+    MFMA kernels' FLOP counts (GUIDE_FWD_FLOP / GUIDE_TRAIN_FLOP per row) for a
+    kernel trace of this run."""
     from renderx_amd import api, scenes
 
-    sc = scenes.media_scene(res, res, max_depth=16, freq=158, glass_freq=16)
+    if scene == "glossy":
+        sc = scenes.icosphere_scene(res, res, max_depth=10)
+        desc = f"Phong-glossy icosphere {sc.n_tris} tris, {res}x{res} px, max depth 10"
+    else:
+        sc = scenes.media_scene(res, res, max_depth=16, freq=158, glass_freq=16)
+        desc = f"media+glass+Phong sphere {sc.n_tris} tris, {res}x{res} px, max depth 16"
     ctx = api.Context(0)
-    out = {"scene": f"media+glass+Phong sphere {sc.n_tris} tris, {res}x{res} px, max depth 16",
+    out = {"scene": desc,
            "calls": calls, "spp_per_call": chunk, "fwd_flop_per_row": GUIDE_FWD_FLOP,
            "train_flop_per_row": GUIDE_TRAIN_FLOP}
     for name, mode in [("guided", api.MODE_GUIDED), ("nee", api.MODE_NEE)]:
@@ -411,8 +416,9 @@ def main():
     ap.add_argument("--no-bpt", action="store_true", help="skip the bidirectional tracer")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (reference kernel for PMC byte counters)")
-    ap.add_argument("--guided-profile", action="store_true",
-                    help="only the guided render of the equal-time study (for a kernel trace); prints its JSON")
+    ap.add_argument("--guided-profile", nargs="?", const="media", choices=["media", "glossy"],
+                    help="only the guided render of an equal-time study (media, the default, or glossy), for a "
+                         "kernel trace; prints its JSON")
     ap.add_argument("--batch-profile", action="store_true",
                     help="only one graph-replayed render of the 64-scene batch (for a kernel trace); prints its JSON")
     args = ap.parse_args()
@@ -420,7 +426,7 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only"}))
         sys.exit(2)
     if args.guided_profile:
-        print(json.dumps(guided_profile()), flush=True)
+        print(json.dumps(guided_profile(args.guided_profile)), flush=True)
         return
     if args.batch_profile:
         print(json.dumps(batch64_profile()), flush=True)

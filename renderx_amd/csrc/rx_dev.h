@@ -129,8 +129,11 @@ __host__ __device__ __forceinline__ float med3f(float a, float b, float c) {
 #ifndef RX_CERTAIN_HIT
 #define RX_CERTAIN_HIT 1  // 0: only the miss half (fewer registers in the traversal loop)
 #endif
-__host__ __device__ __forceinline__ int slab_classify(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
-                                                      Vec3 beta, float& tnear) {
+// slab_classify's two verdicts as lane masks: certain miss, certain hit
+// (never both); neither means the exact predicate must decide.
+__host__ __device__ __forceinline__ void slab_verdicts(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
+                                                       Vec3 beta, float& tnear, bool& certain_miss,
+                                                       bool& certain_hit) {
   // the set-up (|d_a| >= 2^-30, finite boxes) keeps NaN out of every value
   // below, so min/max equal the compare-selects of slab_tnear
   const float U8 = 8.0f * 0x1p-24f, U24 = 24.0f * 0x1p-24f;
@@ -142,25 +145,32 @@ __host__ __device__ __forceinline__ int slab_classify(const float lo[3], const f
   const float ez = fminf(t0z, t1z), xz = fmaxf(t0z, t1z);
   const float t = fmaxf(ex, fmaxf(ey, ez));
   tnear = t;
+  // The rounding bound of each slab end, U8 |end| + beta_a, rounded once;
+  // the end is widened outwards by it for the miss test and narrowed inwards
+  // for the hit test, each with one more rounding: no larger error than
+  // rounding end -+ U8 |end| and then subtracting / adding beta_a.
+  const float wex = fmaf(U8, fabsf(ex), beta.x), wxx = fmaf(U8, fabsf(xx), beta.x);
+  const float wey = fmaf(U8, fabsf(ey), beta.y), wxy = fmaf(U8, fabsf(xy), beta.y);
+  const float wez = fmaf(U8, fabsf(ez), beta.z), wxz = fmaf(U8, fabsf(xz), beta.z);
   // widened slab ends: true e_a - beta_a >= le_a, true x_a + beta_a <= hx_a
-  const float lx = fmaf(-U8, fabsf(ex), ex) - beta.x, hxx = fmaf(U8, fabsf(xx), xx) + beta.x;
-  const float ly = fmaf(-U8, fabsf(ey), ey) - beta.y, hxy = fmaf(U8, fabsf(xy), xy) + beta.y;
-  const float lz = fmaf(-U8, fabsf(ez), ez) - beta.z, hxz = fmaf(U8, fabsf(xz), xz) + beta.z;
-  const float L = fmaxf(fmaxf(0.0f, lx), fmaxf(ly, lz));
-  const float R = fminf(hxx, fminf(hxy, hxz));
+  const float L = fmaxf(fmaxf(0.0f, ex - wex), fmaxf(ey - wey, ez - wez));
+  const float R = fminf(xx + wxx, fminf(xy + wxy, xz + wxz));
   const bool miss = L * (1.0f - 16.0f * 0x1p-24f) > R * (1.0f + 16.0f * 0x1p-24f);
+  certain_miss = miss;
+  certain_hit = false;
 #if RX_CERTAIN_HIT == 0
-  return miss ? kBoxMiss : kBoxUnknown;
+  return;
 #endif
-  // candidate face: the entry face if the entry is clearly ahead, else the
-  // exit face (the first axis whose entry, or exit, equals it); its point
-  // must lie inside the other two slabs with the rounding bounds to spare.
-  // Every candidate bound is computed before the selects, which keeps the
-  // loop free of branches.
-  const float eux = fmaf(U8, fabsf(ex), ex) + beta.x, xdx = fmaf(-U8, fabsf(xx), xx) - beta.x;
-  const float euy = fmaf(U8, fabsf(ey), ey) + beta.y, xdy = fmaf(-U8, fabsf(xy), xy) - beta.y;
-  const float euz = fmaf(U8, fabsf(ez), ez) + beta.z, xdz = fmaf(-U8, fabsf(xz), xz) - beta.z;
-  const bool entry = fmaf(-U8, fabsf(t), t) > 0.0f;
+  // candidate face: the entry face if the entry is ahead (t > 0: the slab
+  // end has the sign of (plane - o) / d, so the real entry is ahead too),
+  // else the exit face (the first axis whose entry, or exit, equals it); its
+  // point must lie inside the other two slabs with the rounding bounds to
+  // spare.  Every candidate bound is computed before the selects, which
+  // keeps the loop free of branches.
+  const float eux = ex + wex, xdx = xx - wxx;
+  const float euy = ey + wey, xdy = xy - wxy;
+  const float euz = ez + wez, xdz = xz - wxz;
+  const bool entry = t > 0.0f;
   const float tx = fminf(xx, fminf(xy, xz));
   const float tf = entry ? t : tx;
   const float e_yz = fmaxf(euy, euz), e_xz = fmaxf(eux, euz), e_xy = fmaxf(eux, euy);
@@ -168,8 +178,14 @@ __host__ __device__ __forceinline__ int slab_classify(const float lo[3], const f
   const bool fx = tf == (entry ? ex : xx), fy = tf == (entry ? ey : xy);
   const float eo = fx ? e_yz : (fy ? e_xz : e_xy);
   const float xo = fx ? x_yz : (fy ? x_xz : x_xy);
-  const bool ok = fmaf(-U8, fabsf(tf), tf) > 0.0f && eo <= tf * (1.0f - U24) && tf * (1.0f + U24) <= xo;
-  return miss ? kBoxMiss : (ok ? kBoxHit : kBoxUnknown);
+  const bool ok = tf > 0.0f && eo <= tf * (1.0f - U24) && tf * (1.0f + U24) <= xo;
+  certain_hit = ok && !miss;
+}
+__host__ __device__ __forceinline__ int slab_classify(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
+                                                      Vec3 beta, float& tnear) {
+  bool m, h;
+  slab_verdicts(lo, hi, o, inv, beta, tnear, m, h);
+  return m ? kBoxMiss : (h ? kBoxHit : kBoxUnknown);
 }
 // The miss half of slab_classify.
 __host__ __device__ __forceinline__ bool slab_certain_miss(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
@@ -324,13 +340,14 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
     bool hl, hr;
     float tl = 0.0f, tr = 0.0f;
     if (s.fast) {  // the exact predicate only where the slab filter cannot decide
-      const int cl = slab_classify(llo, lhi, s.o, s.inv, s.beta, tl);
-      const int cr = slab_classify(rlo, rhi, s.o, s.inv, s.beta, tr);
+      bool ml, cl, mr, cr;
+      slab_verdicts(llo, lhi, s.o, s.inv, s.beta, tl, ml, cl);
+      slab_verdicts(rlo, rhi, s.o, s.inv, s.beta, tr, mr, cr);
       // a child the ordered traversal culls (entry beyond the cull distance)
       // needs no exact test: its answer would not be used
       const float cull = EXHAUSTIVE ? INF : fminf(s.has ? s.best * 1.001f : INF, s.limit);
-      hl = cl == kBoxHit || (cl == kBoxUnknown && (EXHAUSTIVE || tl <= cull) && box_hit(llo, lhi, s.o, s.d));
-      hr = cr == kBoxHit || (cr == kBoxUnknown && (EXHAUSTIVE || tr <= cull) && box_hit(rlo, rhi, s.o, s.d));
+      hl = cl || (!ml && (EXHAUSTIVE || tl <= cull) && box_hit(llo, lhi, s.o, s.d));
+      hr = cr || (!mr && (EXHAUSTIVE || tr <= cull) && box_hit(rlo, rhi, s.o, s.d));
     } else {
       hl = box_hit(llo, lhi, s.o, s.d);
       hr = box_hit(rlo, rhi, s.o, s.d);
@@ -599,10 +616,11 @@ __device__ __forceinline__ void trace_packet(const DevTree& T, const TreeRoot& r
         const float thr = s.has ? s.best * 1.001f : INF;
         bool hl, hr;
         if (s.fast) {
-          const int cl = slab_classify(llo, lhi, s.o, s.inv, s.beta, tl);
-          const int cr = slab_classify(rlo, rhi, s.o, s.inv, s.beta, tr);
-          hl = cl == kBoxHit || (cl == kBoxUnknown && tl <= thr && box_hit(llo, lhi, s.o, s.d));
-          hr = cr == kBoxHit || (cr == kBoxUnknown && tr <= thr && box_hit(rlo, rhi, s.o, s.d));
+          bool ml, cl, mr, cr;
+          slab_verdicts(llo, lhi, s.o, s.inv, s.beta, tl, ml, cl);
+          slab_verdicts(rlo, rhi, s.o, s.inv, s.beta, tr, mr, cr);
+          hl = cl || (!ml && tl <= thr && box_hit(llo, lhi, s.o, s.d));
+          hr = cr || (!mr && tr <= thr && box_hit(rlo, rhi, s.o, s.d));
         } else {
           hl = box_hit(llo, lhi, s.o, s.d);
           hr = box_hit(rlo, rhi, s.o, s.d);
