@@ -250,9 +250,10 @@ float intersect_exhaustive(const Tree& t, const Ray& r, unsigned& tid, const Leg
 
 // ---- canonical ordered traversal -----------------------------------------
 // Slab entry distance with per-ray reciprocal; an ordering key only.  Same
-// operations (one correctly rounded reciprocal per axis, sub then mul,
-// compare-select) as the HIP traversal kernel so both produce the same key.
-static inline float slab_tnear(V3 lo, V3 hi, const Ray& r, const float inv[3]) {
+// operations (one correctly rounded reciprocal per axis, oi = o * inv rounded,
+// each slab end one fused multiply-add fma(plane, inv, -oi), compare-select)
+// as the HIP traversal kernel so both produce the same key.
+static inline float slab_tnear(V3 lo, V3 hi, const Ray& r, const float inv[3], const float oi[3]) {
   const float INF = std::numeric_limits<float>::infinity();
   float tn[3];
   for (int a = 0; a < 3; a++) {
@@ -261,8 +262,8 @@ static inline float slab_tnear(V3 lo, V3 hi, const Ray& r, const float inv[3]) {
       t0 = -INF;
       t1 = INF;
     } else {
-      t0 = (lo[a] - r.o[a]) * inv[a];
-      t1 = (hi[a] - r.o[a]) * inv[a];
+      t0 = std::fmaf(lo[a], inv[a], -oi[a]);
+      t1 = std::fmaf(hi[a], inv[a], -oi[a]);
     }
     tn[a] = t0 < t1 ? t0 : t1;
   }
@@ -270,6 +271,19 @@ static inline float slab_tnear(V3 lo, V3 hi, const Ray& r, const float inv[3]) {
   t = tn[1] > t ? tn[1] : t;
   t = tn[2] > t ? tn[2] : t;
   return t;
+}
+// The absolute term of every cull threshold: twice the rounding bound
+// u |oi_a| of the keys' slab ends over the axes with d_a != 0 (INF when one
+// is not finite), as the HIP traversal computes it.
+static inline float cull_margin(const Ray& r, const float oi[3]) {
+  const float INF = std::numeric_limits<float>::infinity();
+  float m = 0.0f;
+  for (int a = 0; a < 3; a++) {
+    float x = r.d[a] != 0 ? std::fabs(oi[a]) : 0.0f;
+    x = x <= INF ? x : INF;
+    m = x > m ? x : m;
+  }
+  return m * (2.0f * 0x1p-24f);
 }
 
 float intersect_ordered(const Tree& t, const Ray& r, unsigned& tid, const Legality* cond,
@@ -279,8 +293,10 @@ float intersect_ordered(const Tree& t, const Ray& r, unsigned& tid, const Legali
   // first in leaf wins (:305)
   const std::vector<int>& rank = t.leaf_rank;
   const float INF = std::numeric_limits<float>::infinity();
-  float inv[3];
+  float inv[3], oi[3];
   for (int a = 0; a < 3; a++) inv[a] = 1.0f / r.d[a];
+  for (int a = 0; a < 3; a++) oi[a] = r.o[a] * inv[a];
+  const float cm = cull_margin(r, oi);
   bool has = false;
   float best = -1;
   int bestRank = -1;
@@ -310,8 +326,8 @@ float intersect_ordered(const Tree& t, const Ray& r, unsigned& tid, const Legali
       if (st) st->boxes += 2;
       bool hl = box_intersect(L.lo, L.hi, r) >= 0;
       bool hr = box_intersect(R.lo, R.hi, r) >= 0;
-      float tl = slab_tnear(L.lo, L.hi, r, inv), trr = slab_tnear(R.lo, R.hi, r, inv);
-      float thr = has ? best * 1.001f : INF;
+      float tl = slab_tnear(L.lo, L.hi, r, inv, oi), trr = slab_tnear(R.lo, R.hi, r, inv, oi);
+      float thr = has ? std::fmaf(best, 1.001f, cm) : INF;
       bool vl = hl && tl <= thr, vr = hr && trr <= thr;
       if (vl && vr) {
         if (trr < tl) {
@@ -335,7 +351,7 @@ float intersect_ordered(const Tree& t, const Ray& r, unsigned& tid, const Legali
     while (!stack.empty()) {
       E e = stack.back();
       stack.pop_back();
-      float thr = has ? best * 1.001f : INF;
+      float thr = has ? std::fmaf(best, 1.001f, cm) : INF;
       if (e.tn <= thr) {
         cur = e.n;
         found = true;

@@ -85,16 +85,39 @@ __host__ __device__ __forceinline__ bool box_hit(const float lo[3], const float 
   return face_pass(hi[2], o.z, d.z, o.x, d.x, lo[0], hi[0], o.y, d.y, lo[1], hi[1]);
 }
 
+// Per-ray constants of the slab computations: inv = 1 / d (correctly
+// rounded) and oi = o * inv (rounded), so that a slab end is one FMA,
+// t = fma(plane, inv, -oi), instead of (plane - o) * inv.  Besides its
+// relative rounding (2u), t then carries an absolute error of at most
+// u |oi_a|; slab_cull_margin bounds it for the ordered traversal's culling,
+// and the filter's beta_a (16u B_a / |d_a| >= 16u |oi_a|) covers it.
+__host__ __device__ __forceinline__ Vec3 slab_oi(Vec3 o, Vec3 inv) { return v3(o.x * inv.x, o.y * inv.y, o.z * inv.z); }
+// Absolute margin added to every cull threshold: twice the largest |oi_a|
+// rounding bound over the axes the key uses (d_a != 0); INF (no culling)
+// when one is not finite.
+__host__ __device__ __forceinline__ float slab_cull_margin(Vec3 d, Vec3 oi) {
+  const float INF = __builtin_inff();
+  float ax = d.x != 0 ? fabsf(oi.x) : 0.0f, ay = d.y != 0 ? fabsf(oi.y) : 0.0f, az = d.z != 0 ? fabsf(oi.z) : 0.0f;
+  ax = ax <= INF ? ax : INF;
+  ay = ay <= INF ? ay : INF;
+  az = az <= INF ? az : INF;
+  float m = ax;
+  m = ay > m ? ay : m;
+  m = az > m ? az : m;
+  return m * (2.0f * 0x1p-24f);
+}
+
 // Slab entry distance with per-ray reciprocals: ordering key only (not a hit
-// test).  Sub then mul, compare-select; identical to the oracle's key.
-__host__ __device__ __forceinline__ float slab_tnear(const float lo[3], const float hi[3], Vec3 o, Vec3 d, Vec3 inv) {
+// test).  One FMA per slab end (slab_oi), compare-select; identical to the
+// oracle's key.
+__host__ __device__ __forceinline__ float slab_tnear(const float lo[3], const float hi[3], Vec3 oi, Vec3 d, Vec3 inv) {
   const float INF = __builtin_inff();
   float t0, t1, a, b, c;
-  if (d.x == 0) { t0 = -INF; t1 = INF; } else { t0 = (lo[0] - o.x) * inv.x; t1 = (hi[0] - o.x) * inv.x; }
+  if (d.x == 0) { t0 = -INF; t1 = INF; } else { t0 = fmaf(lo[0], inv.x, -oi.x); t1 = fmaf(hi[0], inv.x, -oi.x); }
   a = t0 < t1 ? t0 : t1;
-  if (d.y == 0) { t0 = -INF; t1 = INF; } else { t0 = (lo[1] - o.y) * inv.y; t1 = (hi[1] - o.y) * inv.y; }
+  if (d.y == 0) { t0 = -INF; t1 = INF; } else { t0 = fmaf(lo[1], inv.y, -oi.y); t1 = fmaf(hi[1], inv.y, -oi.y); }
   b = t0 < t1 ? t0 : t1;
-  if (d.z == 0) { t0 = -INF; t1 = INF; } else { t0 = (lo[2] - o.z) * inv.z; t1 = (hi[2] - o.z) * inv.z; }
+  if (d.z == 0) { t0 = -INF; t1 = INF; } else { t0 = fmaf(lo[2], inv.z, -oi.z); t1 = fmaf(hi[2], inv.z, -oi.z); }
   c = t0 < t1 ? t0 : t1;
   float t = a;
   t = b > t ? b : t;
@@ -109,8 +132,10 @@ __host__ __device__ __forceinline__ float slab_tnear(const float lo[3], const fl
 // scene box and the ray origin):
 // in t units e_a - 4u t* - beta_a <= t* <= x_a + 4u t* + beta_a with
 // beta_a = 4u B_a / |d_a| (+ an absolute floor).  The slab ends e_a, x_a are
-// computed here with reciprocals (relative error <= 3u, bounded by 8u with
-// this function's own rounding), so when no t >= 0 satisfies the widened
+// computed here with reciprocals as fma(plane, inv, -oi) (relative error <=
+// 2u, bounded by 8u with this function's own rounding, plus an absolute
+// u |oi_a| <= u B_a / |d_a| that beta's 16u B_a / |d_a| covers with the face
+// check's 4u B_a / |d_a|), so when no t >= 0 satisfies the widened
 // bounds, box_hit is certainly false (kBoxMiss).  Conversely, if the ray
 // enters the box (or, from inside, leaves it) through the face of axis A at
 // t* > 0 and t* lies inside every other slab by more than those bounds, the
@@ -131,15 +156,15 @@ __host__ __device__ __forceinline__ float med3f(float a, float b, float c) {
 #endif
 // slab_classify's two verdicts as lane masks: certain miss, certain hit
 // (never both); neither means the exact predicate must decide.
-__host__ __device__ __forceinline__ void slab_verdicts(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
+__host__ __device__ __forceinline__ void slab_verdicts(const float lo[3], const float hi[3], Vec3 oi, Vec3 inv,
                                                        Vec3 beta, float& tnear, bool& certain_miss,
                                                        bool& certain_hit) {
   // the set-up (|d_a| >= 2^-30, finite boxes) keeps NaN out of every value
   // below, so min/max equal the compare-selects of slab_tnear
   const float U8 = 8.0f * 0x1p-24f, U24 = 24.0f * 0x1p-24f;
-  const float t0x = (lo[0] - o.x) * inv.x, t1x = (hi[0] - o.x) * inv.x;
-  const float t0y = (lo[1] - o.y) * inv.y, t1y = (hi[1] - o.y) * inv.y;
-  const float t0z = (lo[2] - o.z) * inv.z, t1z = (hi[2] - o.z) * inv.z;
+  const float t0x = fmaf(lo[0], inv.x, -oi.x), t1x = fmaf(hi[0], inv.x, -oi.x);
+  const float t0y = fmaf(lo[1], inv.y, -oi.y), t1y = fmaf(hi[1], inv.y, -oi.y);
+  const float t0z = fmaf(lo[2], inv.z, -oi.z), t1z = fmaf(hi[2], inv.z, -oi.z);
   const float ex = fminf(t0x, t1x), xx = fmaxf(t0x, t1x);
   const float ey = fminf(t0y, t1y), xy = fmaxf(t0y, t1y);
   const float ez = fminf(t0z, t1z), xz = fmaxf(t0z, t1z);
@@ -181,16 +206,16 @@ __host__ __device__ __forceinline__ void slab_verdicts(const float lo[3], const 
   const bool ok = tf > 0.0f && eo <= tf * (1.0f - U24) && tf * (1.0f + U24) <= xo;
   certain_hit = ok && !miss;
 }
-__host__ __device__ __forceinline__ int slab_classify(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
+__host__ __device__ __forceinline__ int slab_classify(const float lo[3], const float hi[3], Vec3 oi, Vec3 inv,
                                                       Vec3 beta, float& tnear) {
   bool m, h;
-  slab_verdicts(lo, hi, o, inv, beta, tnear, m, h);
+  slab_verdicts(lo, hi, oi, inv, beta, tnear, m, h);
   return m ? kBoxMiss : (h ? kBoxHit : kBoxUnknown);
 }
 // The miss half of slab_classify.
-__host__ __device__ __forceinline__ bool slab_certain_miss(const float lo[3], const float hi[3], Vec3 o, Vec3 inv,
+__host__ __device__ __forceinline__ bool slab_certain_miss(const float lo[3], const float hi[3], Vec3 oi, Vec3 inv,
                                                            Vec3 beta, float& tnear) {
-  return slab_classify(lo, hi, o, inv, beta, tnear) == kBoxMiss;
+  return slab_classify(lo, hi, oi, inv, beta, tnear) == kBoxMiss;
 }
 // Per-ray set-up of slab_certain_miss; false when the filter must not be used.
 // B_a bounds every coordinate the face check adds or subtracts on axis a:
@@ -258,7 +283,8 @@ struct TraceStats {
 // Traversal state of one ray, so that a lane can run it in bounded trips
 // (descend to a leaf, test the leaf, pop) and take a new ray when it ends.
 struct TraceState {
-  Vec3 o, d, inv, beta;
+  Vec3 o, d, inv, oi, beta;
+  float cm;     // slab_cull_margin: absolute term of every cull threshold
   bool fast, has;
   float best;
   float limit;  // subtrees whose entry key exceeds it are culled (INF: none)
@@ -301,11 +327,13 @@ __device__ __forceinline__ bool trace_begin(const TreeRoot& R, TraceState& s, Ve
   s.sp = 0;
   s.ref = R.ref;
   s.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  s.oi = slab_oi(o, s.inv);
+  s.cm = slab_cull_margin(d, s.oi);
   s.beta = v3(0, 0, 0);
   s.fast = slab_filter_setup(R.lo, R.hi, o, d, s.inv, s.beta);
   if (s.fast) {  // the root box as any other: the exact predicate only if undecided
     float tn;
-    const int c = slab_classify(R.lo, R.hi, o, s.inv, s.beta, tn);
+    const int c = slab_classify(R.lo, R.hi, s.oi, s.inv, s.beta, tn);
     if (c != kBoxUnknown) return c == kBoxHit;
   }
   return box_hit(R.lo, R.hi, o, d);
@@ -341,19 +369,19 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
     float tl = 0.0f, tr = 0.0f;
     if (s.fast) {  // the exact predicate only where the slab filter cannot decide
       bool ml, cl, mr, cr;
-      slab_verdicts(llo, lhi, s.o, s.inv, s.beta, tl, ml, cl);
-      slab_verdicts(rlo, rhi, s.o, s.inv, s.beta, tr, mr, cr);
+      slab_verdicts(llo, lhi, s.oi, s.inv, s.beta, tl, ml, cl);
+      slab_verdicts(rlo, rhi, s.oi, s.inv, s.beta, tr, mr, cr);
       // a child the ordered traversal culls (entry beyond the cull distance)
       // needs no exact test: its answer would not be used
-      const float cull = EXHAUSTIVE ? INF : fminf(s.has ? s.best * 1.001f : INF, s.limit);
+      const float cull = EXHAUSTIVE ? INF : fminf(s.has ? fmaf(s.best, 1.001f, s.cm) : INF, s.limit + s.cm);
       hl = cl || (!ml && (EXHAUSTIVE || tl <= cull) && box_hit(llo, lhi, s.o, s.d));
       hr = cr || (!mr && (EXHAUSTIVE || tr <= cull) && box_hit(rlo, rhi, s.o, s.d));
     } else {
       hl = box_hit(llo, lhi, s.o, s.d);
       hr = box_hit(rlo, rhi, s.o, s.d);
       if (!EXHAUSTIVE) {
-        tl = slab_tnear(llo, lhi, s.o, s.d, s.inv);
-        tr = slab_tnear(rlo, rhi, s.o, s.d, s.inv);
+        tl = slab_tnear(llo, lhi, s.oi, s.d, s.inv);
+        tr = slab_tnear(rlo, rhi, s.oi, s.d, s.inv);
       }
     }
     int32_t next = INT32_MIN;
@@ -368,7 +396,7 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
       } else if (hl) next = r.x;
       else if (hr) next = r.y;
     } else {
-      const float thr = fminf(s.has ? s.best * 1.001f : INF, s.limit);
+      const float thr = fminf(s.has ? fmaf(s.best, 1.001f, s.cm) : INF, s.limit + s.cm);
       const bool vl = hl && tl <= thr, vr = hr && tr <= thr;
       if (vl && vr) {
         const int32_t nearRef = (tr < tl) ? r.y : r.x;
@@ -418,7 +446,7 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
     s.sp--;
     const uint2 e = (s.sp < LDS_DEPTH) ? lds_stack[s.sp * stride] : spill[s.sp - LDS_DEPTH];
     const float key = __uint_as_float(e.y);
-    if (EXHAUSTIVE || (s.has ? key <= fminf(s.best * 1.001f, s.limit) : !(key > s.limit))) {
+    if (EXHAUSTIVE || (s.has ? key <= fminf(fmaf(s.best, 1.001f, s.cm), s.limit + s.cm) : !(key > s.limit + s.cm))) {
       ref = (int32_t)e.x;
       break;
     }
@@ -613,19 +641,19 @@ __device__ __forceinline__ void trace_packet(const DevTree& T, const TreeRoot& r
         const float llo[3] = {a.x, a.y, a.z}, lhi[3] = {a.w, b.x, b.y};
         const float rlo[3] = {b.z, b.w, c.x}, rhi[3] = {c.y, c.z, c.w};
         if (STATS) st->boxes += 2;
-        const float thr = s.has ? s.best * 1.001f : INF;
+        const float thr = s.has ? fmaf(s.best, 1.001f, s.cm) : INF;
         bool hl, hr;
         if (s.fast) {
           bool ml, cl, mr, cr;
-          slab_verdicts(llo, lhi, s.o, s.inv, s.beta, tl, ml, cl);
-          slab_verdicts(rlo, rhi, s.o, s.inv, s.beta, tr, mr, cr);
+          slab_verdicts(llo, lhi, s.oi, s.inv, s.beta, tl, ml, cl);
+          slab_verdicts(rlo, rhi, s.oi, s.inv, s.beta, tr, mr, cr);
           hl = cl || (!ml && tl <= thr && box_hit(llo, lhi, s.o, s.d));
           hr = cr || (!mr && tr <= thr && box_hit(rlo, rhi, s.o, s.d));
         } else {
           hl = box_hit(llo, lhi, s.o, s.d);
           hr = box_hit(rlo, rhi, s.o, s.d);
-          tl = slab_tnear(llo, lhi, s.o, s.d, s.inv);
-          tr = slab_tnear(rlo, rhi, s.o, s.d, s.inv);
+          tl = slab_tnear(llo, lhi, s.oi, s.d, s.inv);
+          tr = slab_tnear(rlo, rhi, s.oi, s.d, s.inv);
         }
         vl = hl && tl <= thr;
         vr = hr && tr <= thr;

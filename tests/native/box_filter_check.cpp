@@ -110,7 +110,8 @@ int main(int argc, char** argv) {
       if (!slab_filter_setup(rlo, rhi, o, d, inv, beta)) continue;
       fast++;
       float tn;
-      const int cls = slab_classify(lo, hi, o, inv, beta, tn);
+      const Vec3 oi = slab_oi(o, inv);
+      const int cls = slab_classify(lo, hi, oi, inv, beta, tn);
       const bool miss = cls == kBoxMiss;
       if (cls == kBoxHit) {
         certain_hit++;
@@ -120,7 +121,7 @@ int main(int argc, char** argv) {
           return 1;
         }
       }
-      const float tref = slab_tnear(lo, hi, o, d, inv);
+      const float tref = slab_tnear(lo, hi, oi, d, inv);
       if (!(tn == tref || (std::isnan(tn) && std::isnan(tref)))) {
         printf("key mismatch fam %d: %.9g vs %.9g\n", fam, tn, tref);
         return 1;
