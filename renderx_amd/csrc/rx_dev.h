@@ -180,7 +180,9 @@ __host__ __device__ __forceinline__ void slab_verdicts(const float lo[3], const 
   // widened slab ends: true e_a - beta_a >= le_a, true x_a + beta_a <= hx_a
   const float L = fmaxf(fmaxf(0.0f, ex - wex), fmaxf(ey - wey, ez - wez));
   const float R = fminf(xx + wxx, fminf(xy + wxy, xz + wxz));
-  const bool miss = L * (1.0f - 16.0f * 0x1p-24f) > R * (1.0f + 16.0f * 0x1p-24f);
+  // L (1 - 16u) > R (1 + 16u), with one rounding: L >= 0, and for R >= 0
+  // (1 + 16u) / (1 - 16u) < 1 + 33u, so this is the stronger condition
+  const bool miss = L > R * (1.0f + 33.0f * 0x1p-24f);
   certain_miss = miss;
   certain_hit = false;
 #if RX_CERTAIN_HIT == 0
@@ -190,19 +192,20 @@ __host__ __device__ __forceinline__ void slab_verdicts(const float lo[3], const 
   // end has the sign of (plane - o) / d, so the real entry is ahead too),
   // else the exit face (the first axis whose entry, or exit, equals it); its
   // point must lie inside the other two slabs with the rounding bounds to
-  // spare.  Every candidate bound is computed before the selects, which
-  // keeps the loop free of branches.
+  // spare: the candidate axis's own bounds are replaced by -inf / +inf and
+  // the three are reduced, which keeps the loop free of branches.
+  const float INF = __builtin_inff();
   const float eux = ex + wex, xdx = xx - wxx;
   const float euy = ey + wey, xdy = xy - wxy;
   const float euz = ez + wez, xdz = xz - wxz;
   const bool entry = t > 0.0f;
   const float tx = fminf(xx, fminf(xy, xz));
   const float tf = entry ? t : tx;
-  const float e_yz = fmaxf(euy, euz), e_xz = fmaxf(eux, euz), e_xy = fmaxf(eux, euy);
-  const float x_yz = fminf(xdy, xdz), x_xz = fminf(xdx, xdz), x_xy = fminf(xdx, xdy);
-  const bool fx = tf == (entry ? ex : xx), fy = tf == (entry ? ey : xy);
-  const float eo = fx ? e_yz : (fy ? e_xz : e_xy);
-  const float xo = fx ? x_yz : (fy ? x_xz : x_xy);
+  const bool fx = tf == (entry ? ex : xx);
+  const bool fy = !fx && tf == (entry ? ey : xy);
+  const bool fz = !fx && !fy;
+  const float eo = fmaxf(fx ? -INF : eux, fmaxf(fy ? -INF : euy, fz ? -INF : euz));
+  const float xo = fminf(fx ? INF : xdx, fminf(fy ? INF : xdy, fz ? INF : xdz));
   const bool ok = tf > 0.0f && eo <= tf * (1.0f - U24) && tf * (1.0f + U24) <= xo;
   certain_hit = ok && !miss;
 }
@@ -357,6 +360,19 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
                                            int spill_depth, TraceStats* st, bool* overflow) {
   const float INF = __builtin_inff();
   const int stride = blockDim.x;
+  // The lane's stack column lies in LDS: an LDS-qualified pointer keeps its
+  // accesses ds_read / ds_write (a pointer chosen between the column and the
+  // global spill would be a flat access), and the column offset of the top
+  // entry, sp * stride, moves by one stride per push and pop.
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(3))) unsigned long long LdsU64;
+  LdsU64* const col = (LdsU64*)lds_stack;
+#else
+  unsigned long long* const col = reinterpret_cast<unsigned long long*>(lds_stack);
+#endif
+  // an entry (ref, key) as one 64-bit LDS word: ref in the low half
+  auto pack = [](uint2 e) { return (unsigned long long)e.x | ((unsigned long long)e.y << 32); };
+  int top = s.sp * stride;
   int32_t ref = s.ref;
   while (ref >= 0) {
     const float4* np = T.nodes + 4 * (size_t)ref;
@@ -388,10 +404,11 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
     if (EXHAUSTIVE) {
       if (hl && hr) {
         const uint2 e = make_uint2((uint32_t)r.y, 0u);
-        if (s.sp < LDS_DEPTH) lds_stack[s.sp * stride] = e;
+        if (s.sp < LDS_DEPTH) col[top] = pack(e);
         else if (s.sp - LDS_DEPTH < spill_depth) spill[s.sp - LDS_DEPTH] = e;
         else { *overflow = true; s.has = false; return true; }
         s.sp++;
+        top += stride;
         next = r.x;
       } else if (hl) next = r.x;
       else if (hr) next = r.y;
@@ -403,10 +420,11 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
         const int32_t farRef = (tr < tl) ? r.x : r.y;
         const float farT = (tr < tl) ? tl : tr;
         const uint2 e = make_uint2((uint32_t)farRef, __float_as_uint(farT));
-        if (s.sp < LDS_DEPTH) lds_stack[s.sp * stride] = e;
+        if (s.sp < LDS_DEPTH) col[top] = pack(e);
         else if (s.sp - LDS_DEPTH < spill_depth) spill[s.sp - LDS_DEPTH] = e;
         else { *overflow = true; s.has = false; return true; }
         s.sp++;
+        top += stride;
         next = nearRef;
       } else if (vl) next = r.x;
       else if (vr) next = r.y;
@@ -444,7 +462,18 @@ __device__ __forceinline__ bool trace_trip(const DevTree& T, TraceState& s, uint
   ref = INT32_MIN;
   while (s.sp > 0) {
     s.sp--;
-    const uint2 e = (s.sp < LDS_DEPTH) ? lds_stack[s.sp * stride] : spill[s.sp - LDS_DEPTH];
+    top -= stride;
+    // the LDS entry is read unconditionally (an in-range one below the
+    // spill), the spill entry only where the stack has spilled (a volatile
+    // read, which the compiler cannot merge with the LDS read into one
+    // flat access through a selected pointer)
+    const unsigned long long w = col[s.sp < LDS_DEPTH ? top : (LDS_DEPTH - 1) * stride];
+    uint2 e = make_uint2((uint32_t)w, (uint32_t)(w >> 32));
+    if (s.sp >= LDS_DEPTH) {
+      const volatile uint2* sv = spill + (s.sp - LDS_DEPTH);
+      e.x = sv->x;
+      e.y = sv->y;
+    }
     const float key = __uint_as_float(e.y);
     if (EXHAUSTIVE || (s.has ? key <= fminf(fmaf(s.best, 1.001f, s.cm), s.limit + s.cm) : !(key > s.limit + s.cm))) {
       ref = (int32_t)e.x;
