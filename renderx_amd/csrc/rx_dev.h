@@ -746,15 +746,15 @@ __device__ __forceinline__ void trace_packet(const DevTree& T, const TreeRoot& r
 struct U4 {
   uint32_t x, y, z, w;
 };
-__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
-  return (uint32_t)(((uint64_t)a * b) >> 32);  // v_mul_hi_u32 on the device
-}
 __host__ __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
   const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
   for (int i = 0; i < 10; i++) {
-    uint32_t hi0 = mulhi32(M0, c.x), lo0 = M0 * c.x;
-    uint32_t hi1 = mulhi32(M1, c.z), lo1 = M1 * c.z;
+    // each 32 x 32 -> 64-bit product as one value: one v_mad_u64_u32 on the
+    // device for both halves (a v_mul_hi_u32 and a v_mul_lo_u32 otherwise)
+    const uint64_t p0 = (uint64_t)M0 * c.x, p1 = (uint64_t)M1 * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
     k0 += W0;
     k1 += W1;
