@@ -377,6 +377,11 @@ int rx_guide_set_weights(rx_ctx* ctx, const float* params);
 int rx_guide_get_weights(rx_ctx* ctx, float* params);
 /* Network outputs (48 floats per query) for n positions/directions (3n floats each). */
 int rx_guide_eval(rx_ctx* ctx, uint32_t n, const float* pos, const float* dir, float* out48);
+/* The same queries decoded as RX_MODE_GUIDED's renderer reads them, 52 floats
+ * per query: 8 x (unit mean direction, concentration kappa), 8 x w_j kappa_j /
+ * (2 pi (1 - e^{-2 kappa_j})), 8 cumulative lobe weights (the last is 1), the
+ * selection probability alpha, 3 floats of padding. */
+int rx_guide_eval_mixture(rx_ctx* ctx, uint32_t n, const float* pos, const float* dir, float* out52);
 /* One Adam step on n radiance samples: sampled direction omega (3n), BSDF
  * density at omega, weight F/p_sample; loss = mean of -weight log(alpha q +
  * (1-alpha) p_bsdf).  Optional outputs: mean loss, mean gradient (RX_GUIDE_PARAMS). */

@@ -63,6 +63,7 @@ SIGNATURES = {
     "rx_guide_set_weights": (C.c_int, [P, P]),
     "rx_guide_get_weights": (C.c_int, [P, P]),
     "rx_guide_eval": (C.c_int, [P, U32, P, P, P]),
+    "rx_guide_eval_mixture": (C.c_int, [P, U32, P, P, P]),
     "rx_guide_train_batch": (C.c_int, [P, U32, P, P, P, P, P, P, P]),
 }
 GUIDE_PARAMS = 17648

@@ -146,27 +146,41 @@ __device__ __forceinline__ uint32_t guide_pair(const GuideNet& N, Vec3 p, Vec3 d
 
 // ------------------------------------------------------------------ MFMA tiles
 // Every product of the network has one dimension of 16 rows per wavefront
-// (a tile of queries), mapped onto v_mfma_f32_16x16x16_bf16:
+// (a tile of queries).  Operands are kept as the fragments of K = 16 steps
+// (the layout of v_mfma_f32_16x16x16_bf16):
 //   A fragment: lane l holds A[l%16][4(l/16) .. 4(l/16)+3]
 //   B fragment: lane l holds B[4(l/16) .. 4(l/16)+3][l%16]
 //   D fragment: lane l holds D[4(l/16)+i][l%16], i = 0..3
+// and two consecutive K = 16 steps run as one v_mfma_f32_16x16x32_bf16,
+// gfx950's full-rate BF16 product (the K = 16 form issues at about half the
+// rate): that instruction's lane l holds A[l%16][8(l/16) + j] and
+// B[8(l/16) + j][l%16], j = 0..7, so lane l's K = 16 fragments of steps 2s
+// and 2s + 1 placed side by side give both operands the same permutation of
+// k within the 32-wide step, and the product is the sum of the two steps.
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int kLayerIn[4] = {kGIn, kGHid, kGHid, kGHid};
 constexpr int kLayerOut[4] = {kGHid, kGHid, kGHid, kGOut};
 
+// Four consecutive bf16 values (8-byte aligned) as a fragment, the loaded
+// dwords reinterpreted in place.
 __device__ __forceinline__ v4s ld4(const uint16_t* p) {
-  uint2 v = *reinterpret_cast<const uint2*>(p);
-  v4s r;
-  r[0] = (short)(v.x & 0xffffu);
-  r[1] = (short)(v.x >> 16);
-  r[2] = (short)(v.y & 0xffffu);
-  r[3] = (short)(v.y >> 16);
-  return r;
+  return __builtin_bit_cast(v4s, *reinterpret_cast<const uint2*>(p));
 }
-__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+// Two K = 16 steps (fragments a0, b0 and a1, b1) in one K = 32 product; a
+// product over an odd number of K = 16 steps pads the last with zero
+// fragments.  Every product of the network uses this one instruction: an
+// accumulator passed from a 16x16x32 product to a 16x16x16 one came out wrong
+// in the build tested (the compiler spaced the dependent pair by one
+// instruction), so the two forms are not mixed.
+__device__ __forceinline__ v4f mfma2(v4s a0, v4s a1, v4s b0, v4s b1, v4f c) {
+  const v8s a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+  const v8s b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0,
+                                                 0);
 }
 
 // The forward weights in LDS, [out][in] per layer with rows padded by 8 bf16
@@ -186,6 +200,12 @@ __device__ __forceinline__ void stage_weights(const GuideNet& N, uint16_t* wts) 
           *reinterpret_cast<const uint2*>(N.Wt[L] + (size_t)r * K + col);
     }
   }
+}
+
+// The four layers' biases in LDS, b1 | b2 | b3 | b4 (kBiasCount floats).
+constexpr int kBiasCount = 3 * kGHid + kGOut;
+__device__ __forceinline__ void stage_bias(const GuideNet& N, float* bias) {
+  for (int e = threadIdx.x; e < kBiasCount; e += blockDim.x) bias[e] = N.b[e / kGHid][e % kGHid];
 }
 
 // Inference order of the encoding (k_guide_fwd): at k-step ks the lanes of

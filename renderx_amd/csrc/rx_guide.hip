@@ -31,33 +31,46 @@ namespace rx {
 // row) or, with DECODE, the decoded mixture as kGHead bytes per row (the
 // renderer's form, GuideHead), staged through LDS so that rows are written
 // whole.  Query row r is pos[r * stride], dir[r * stride].
-constexpr int kFwdBlock = 512;  // 8 waves share one copy of the weights
+// 16 waves share one copy of the weights: one block per CU (38.6 KB of
+// weights and 16 x 3 KB of output staging in LDS), 4 waves per SIMD
+constexpr int kFwdBlock = 1024;
 constexpr int kFwdWaves = kFwdBlock / 64;
 
-// One layer: acc[mt] = W[16 mt .. , :] * in + bias for the layer's MT output tiles.
+// One layer: acc[mt] = W[16 mt .. , :] * in + bias for the layer's MT output
+// tiles, the bias loaded as the products' starting accumulator (lane l's four
+// outputs 16 mt + 4(l/16) .. +3 are consecutive, 16-byte aligned floats).
 template <int KS, int MT>
 __device__ __forceinline__ void fwd_layer(const uint16_t* w, int ldw, const float* bias, const v4s* in, v4f* acc,
                                           int lr, int lg) {
+  static_assert(KS % 2 == 0, "layer inputs in whole 32-wide steps");
 #pragma unroll
   for (int mt = 0; mt < MT; mt++) {
-    v4f a = v4f{0, 0, 0, 0};
+    const float4 b4 = *reinterpret_cast<const float4*>(bias + mt * 16 + 4 * lg);
+    v4f a = v4f{b4.x, b4.y, b4.z, b4.w};
+    const uint16_t* wr = w + (mt * 16 + lr) * ldw + 4 * lg;
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) a = mfma(ld4(w + (mt * 16 + lr) * ldw + ks * 16 + 4 * lg), in[ks], a);
-#pragma unroll
-    for (int i = 0; i < 4; i++) a[i] += bias[mt * 16 + 4 * lg + i];
+    for (int ks = 0; ks < KS; ks += 2) a = mfma2(ld4(wr + ks * 16), ld4(wr + ks * 16 + 16), in[ks], in[ks + 1], a);
     acc[mt] = a;
   }
 }
+// max(x, 0) as one v_max_i32 on the bits: every float with the sign bit set
+// (-0 included) is a negative integer and becomes +0, every other float is
+// unchanged.  fmaxf(x, 0) adds a canonicalising v_max_f32 of the MFMA output.
+__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 __device__ __forceinline__ v4s relu_bf16(v4f a) {
-  const uint32_t p0 = bf2_pack(fmaxf(a[0], 0.0f), fmaxf(a[1], 0.0f));
-  const uint32_t p1 = bf2_pack(fmaxf(a[2], 0.0f), fmaxf(a[3], 0.0f));
-  return v4s{(short)(p0 & 0xffffu), (short)(p0 >> 16), (short)(p1 & 0xffffu), (short)(p1 >> 16)};
+  const uint32_t p0 = bf2_pack(relu(a[0]), relu(a[1]));
+  const uint32_t p1 = bf2_pack(relu(a[2]), relu(a[3]));
+  return __builtin_bit_cast(v4s, make_uint2(p0, p1));
 }
-// LDS written by some lanes of a wave, read by others: the wave's LDS
-// operations complete in order once issued.
+// LDS written by some lanes of a wave and read by others of the same wave:
+// a wave's LDS instructions are performed in issue order, so keeping them in
+// program order suffices.  Wavefront-scope fences wait on no counter; a
+// workgroup fence would also wait for the wave's outstanding global loads
+// and stores.
 __device__ __forceinline__ void wave_lds_sync() {
-  __threadfence_block();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <bool DECODE>
@@ -65,17 +78,32 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
                                                          uint32_t stride, const uint32_t* rows, uint32_t cap,
                                                          float* Z4, unsigned char* head) {
   __shared__ __attribute__((aligned(16))) uint16_t wts[kWLds];
+  __shared__ __attribute__((aligned(16))) float bias[kBiasCount];
   __shared__ __attribute__((aligned(16))) float zt[DECODE ? kFwdWaves : 1][16 * kGOut];  // a tile's outputs
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
   const uint32_t M = min(*rows, cap);
   const uint32_t nTiles = (M + 15) / 16;
   if (blockIdx.x * kFwdWaves >= nTiles) return;  // block-uniform
+  // The next tile's query is loaded while this one is computed: issued
+  // before this tile's stores, waiting for it leaves them in flight (the
+  // vector memory counter retires loads and stores in issue order).  Loads
+  // and stores are unconditional so that the waits stay counts, not drains:
+  // indexes past the last tile or row are clamped to it, a row past the
+  // count computes the last row's query and stores the same bytes there.
+  // The first query is loaded before the block barrier, which waits for it,
+  // so the loop is entered with nothing in flight.
+  const uint32_t tstep = gridDim.x * kFwdWaves;
+  uint32_t tile = blockIdx.x * kFwdWaves + wave;
+  auto query_row = [&](uint32_t t) { return (size_t)min(min(t, nTiles - 1) * 16 + (uint32_t)lr, M - 1) * stride; };
+  float4 p4n = pos[query_row(tile)], d4n = dir[query_row(tile)];
   stage_weights_chain(N, wts);
+  stage_bias(N, bias);
   __syncthreads();
-  for (uint32_t tile = blockIdx.x * kFwdWaves + wave; tile < nTiles; tile += gridDim.x * kFwdWaves) {
+  for (; tile < nTiles; tile += tstep) {
     const uint32_t r0 = tile * 16;
-    const uint32_t gr = min(r0 + (uint32_t)lr, M - 1);
-    const float4 p4 = pos[(size_t)gr * stride], d4 = dir[(size_t)gr * stride];
+    const float4 p4 = p4n, d4 = d4n;
+    p4n = pos[query_row(tile + tstep)];
+    d4n = dir[query_row(tile + tstep)];
     // the lane's chain (enc_chain_pair): a position coordinate in [0, 1]
     // through 12 octaves, or the direction's three coordinates through 4 each
     const float pc = lg == 0 ? p4.x : (lg == 1 ? p4.y : p4.z);
@@ -103,22 +131,22 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
     }
     v4f acc[4];
     v4s h[4];
-    fwd_layer<6, 4>(wts + kWOff[0], kGIn + kWPad, N.b[0], in, acc, lr, lg);
+    fwd_layer<6, 4>(wts + kWOff[0], kGIn + kWPad, bias, in, acc, lr, lg);
 #pragma unroll
     for (int k = 0; k < 4; k++) h[k] = relu_bf16(acc[k]);
-    fwd_layer<4, 4>(wts + kWOff[1], kGHid + kWPad, N.b[1], h, acc, lr, lg);
+    fwd_layer<4, 4>(wts + kWOff[1], kGHid + kWPad, bias + kGHid, h, acc, lr, lg);
 #pragma unroll
     for (int k = 0; k < 4; k++) h[k] = relu_bf16(acc[k]);
-    fwd_layer<4, 4>(wts + kWOff[2], kGHid + kWPad, N.b[2], h, acc, lr, lg);
+    fwd_layer<4, 4>(wts + kWOff[2], kGHid + kWPad, bias + 2 * kGHid, h, acc, lr, lg);
 #pragma unroll
     for (int k = 0; k < 4; k++) h[k] = relu_bf16(acc[k]);
-    fwd_layer<4, 3>(wts + kWOff[3], kGHid + kWPad, N.b[3], h, acc, lr, lg);
+    fwd_layer<4, 3>(wts + kWOff[3], kGHid + kWPad, bias + 3 * kGHid, h, acc, lr, lg);
     if (!DECODE) {  // row lr, outputs 16 mt + 4 lg .. +3
-      if (r0 + (uint32_t)lr < M)
+      const size_t orow = min(r0 + (uint32_t)lr, M - 1);
 #pragma unroll
-        for (int mt = 0; mt < 3; mt++)
-          *reinterpret_cast<float4*>(Z4 + (size_t)(r0 + lr) * kGOut + mt * 16 + 4 * lg) =
-              make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
+      for (int mt = 0; mt < 3; mt++)
+        *reinterpret_cast<float4*>(Z4 + orow * kGOut + mt * 16 + 4 * lg) =
+            make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
       continue;
     }
     float* z = zt[DECODE ? wave : 0];
@@ -130,8 +158,7 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
     {  // four lanes per row, two lobes each, written to the row's decoded record directly
       const int r = lane >> 2, sub = lane & 3;
       const float* zr = z + r * kGOut;
-      const bool out = r0 + (uint32_t)r < M;
-      unsigned char* hrow = head + (size_t)(r0 + r) * kGHead;
+      unsigned char* hrow = head + (size_t)min(r0 + (uint32_t)r, M - 1) * kGHead;
       const int j0 = 2 * sub, j1 = 2 * sub + 1;
       const float l0 = zr[5 * j0], l1 = zr[5 * j1];
       float mx = fmaxf(l0, l1);
@@ -158,16 +185,14 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
         const float m0 = zr[5 * j + 1], m1 = zr[5 * j + 2], m2 = zr[5 * j + 3];
         const float il = __builtin_amdgcn_rsqf(fmaxf(m0 * m0 + m1 * m1 + m2 * m2, 1e-12f));
         const float k = __expf(fminf(8.0f, fmaxf(-3.0f, zr[5 * j + 4])));
-        if (out) {
-          reinterpret_cast<float4*>(hrow)[j] = make_float4(m0 * il, m1 * il, m2 * il, k);
-          reinterpret_cast<float*>(hrow + kGHeadCw)[j] =
-              wts2[u] * k * __builtin_amdgcn_rcpf(6.28318530717958647692f * (1.0f - __expf(-2.0f * k)));
-          reinterpret_cast<float*>(hrow + kGHeadCdf)[j] = cdfs[u];
-        }
+        reinterpret_cast<float4*>(hrow)[j] = make_float4(m0 * il, m1 * il, m2 * il, k);
+        reinterpret_cast<float*>(hrow + kGHeadCw)[j] =
+            wts2[u] * k * __builtin_amdgcn_rcpf(6.28318530717958647692f * (1.0f - __expf(-2.0f * k)));
+        reinterpret_cast<float*>(hrow + kGHeadCdf)[j] = cdfs[u];
       }
-      if (sub == 0 && out)
-        *reinterpret_cast<float4*>(hrow + kGHeadAlpha) =
-            make_float4(kAlphaMin + kAlphaSpan / (1.0f + __expf(-zr[40])), 0.0f, 0.0f, 0.0f);
+      // the row's four lanes store the same selection record
+      *reinterpret_cast<float4*>(hrow + kGHeadAlpha) =
+          make_float4(kAlphaMin + kAlphaSpan / (1.0f + __expf(-zr[40])), 0.0f, 0.0f, 0.0f);
     }
     wave_lds_sync();  // the tile's outputs are read before the next tile overwrites them
   }
@@ -175,9 +200,10 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
 
 hipError_t guide_infer(const GuideNet& N, const float4* pos, const float4* dir, uint32_t stride, const uint32_t* rows,
                        uint32_t cap, float* Z4, unsigned char* head, int n_cu, hipStream_t s) {
+  // one block per CU (the LDS of one fits), each grid-striding over the tiles
   const uint32_t tiles = (cap + 15) / 16;
   const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves,
-                                                                   (uint32_t)n_cu * 2));
+                                                                   (uint32_t)n_cu));
   if (head)
     hipLaunchKernelGGL(k_guide_fwd<true>, dim3(blocks), dim3(kFwdBlock), 0, s, N, pos, dir, stride, rows, cap, Z4,
                        head);
@@ -273,6 +299,7 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_train(GuideNet N, TrainRo
                                                              uint32_t cap, float* partial, float* loss_part) {
   __shared__ __attribute__((aligned(16))) uint16_t wts[kWLds];
   __shared__ __attribute__((aligned(16))) uint16_t wtr[kRLds];
+  __shared__ __attribute__((aligned(16))) float bias[kBiasCount];
   __shared__ __attribute__((aligned(16))) uint16_t tiles[kTrainWaves][kTileFeat * 16];
   __shared__ __attribute__((aligned(16))) float zt[kTrainWaves][16 * kGOut];
   __shared__ float lsum[kTrainWaves];
@@ -280,6 +307,7 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_train(GuideNet N, TrainRo
   const uint32_t M = min(*rows, cap);
   const uint32_t nIter = (M + 16 * kTrainWaves - 1) / (16 * kTrainWaves);
   stage_weights(N, wts);
+  stage_bias(N, bias);
   for (int L = 1; L < 4; L++) {  // backward weights, [in][out]
     const int K = kLayerOut[L], ld = K + kRPad;
     for (int e = threadIdx.x; e < kGHid * K / 4; e += blockDim.x) {
@@ -311,16 +339,16 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_train(GuideNet N, TrainRo
     }
     v4f acc[4];
     v4s h1[4], h2[4], h3[4];
-    fwd_layer<6, 4>(wts + kWOff[0], kGIn + kWPad, N.b[0], in, acc, lr, lg);
+    fwd_layer<6, 4>(wts + kWOff[0], kGIn + kWPad, bias, in, acc, lr, lg);
 #pragma unroll
     for (int k = 0; k < 4; k++) h1[k] = relu_bf16(acc[k]);
-    fwd_layer<4, 4>(wts + kWOff[1], kGHid + kWPad, N.b[1], h1, acc, lr, lg);
+    fwd_layer<4, 4>(wts + kWOff[1], kGHid + kWPad, bias + kGHid, h1, acc, lr, lg);
 #pragma unroll
     for (int k = 0; k < 4; k++) h2[k] = relu_bf16(acc[k]);
-    fwd_layer<4, 4>(wts + kWOff[2], kGHid + kWPad, N.b[2], h2, acc, lr, lg);
+    fwd_layer<4, 4>(wts + kWOff[2], kGHid + kWPad, bias + 2 * kGHid, h2, acc, lr, lg);
 #pragma unroll
     for (int k = 0; k < 4; k++) h3[k] = relu_bf16(acc[k]);
-    fwd_layer<4, 3>(wts + kWOff[3], kGHid + kWPad, N.b[3], h3, acc, lr, lg);
+    fwd_layer<4, 3>(wts + kWOff[3], kGHid + kWPad, bias + 3 * kGHid, h3, acc, lr, lg);
 #pragma unroll
     for (int mt = 0; mt < 3; mt++)
       *reinterpret_cast<float4*>(z + lr * kGOut + mt * 16 + 4 * lg) = make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
@@ -411,9 +439,13 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_train(GuideNet N, TrainRo
       const uint16_t* w4 = wtr + kROff[2];
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
-        v4f a = v4f{0, 0, 0, 0};
-#pragma unroll
-        for (int ks = 0; ks < 3; ks++) a = mfma(ld4(w4 + (mt * 16 + lr) * (kGOut + kRPad) + ks * 16 + 4 * lg), dzb[ks], a);
+        // 48 outputs: two 32-wide steps, the second one's upper half zero
+        // in both operands (every product of the network is the same
+        // instruction, so each accumulator chain is one kind of MFMA)
+        const uint16_t* wr = w4 + (mt * 16 + lr) * (kGOut + kRPad) + 4 * lg;
+        const v4s z4 = v4s{0, 0, 0, 0};
+        v4f a = mfma2(ld4(wr), ld4(wr + 16), dzb[0], dzb[1], v4f{0, 0, 0, 0});
+        a = mfma2(ld4(wr + 32), z4, dzb[2], z4, a);
 #pragma unroll
         for (int i = 0; i < 4; i++) dh[mt][i] = (short)f2bf(bf2f((uint16_t)h3[mt][i]) > 0.0f ? a[i] : 0.0f);
         store_frag(tl, kFD3 + 16 * mt, dh[mt], lr, lg);
@@ -427,8 +459,9 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_train(GuideNet N, TrainRo
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
         v4f a = v4f{0, 0, 0, 0};
+        const uint16_t* wr = wl + (mt * 16 + lr) * (kGHid + kRPad) + 4 * lg;
 #pragma unroll
-        for (int ks = 0; ks < 4; ks++) a = mfma(ld4(wl + (mt * 16 + lr) * (kGHid + kRPad) + ks * 16 + 4 * lg), dh[ks], a);
+        for (int ks = 0; ks < 4; ks += 2) a = mfma2(ld4(wr + ks * 16), ld4(wr + ks * 16 + 16), dh[ks], dh[ks + 1], a);
 #pragma unroll
         for (int i = 0; i < 4; i++) nd[mt][i] = (short)f2bf(bf2f((uint16_t)hm[mt][i]) > 0.0f ? a[i] : 0.0f);
         store_frag(tl, (step == 0 ? kFD2 : kFD1) + 16 * mt, nd[mt], lr, lg);
@@ -445,10 +478,12 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_train(GuideNet N, TrainRo
       const int fD = L == 0 ? kFD1 : L == 1 ? kFD2 : L == 2 ? kFD3 : kFDZ;
       const int fH = L == 0 ? kFX : L == 1 ? kFH1 : L == 2 ? kFH2 : kFH3;
       v4f a = dw[j];
+      static_assert(kTrainWaves % 2 == 0, "the block's rows in whole 32-row steps");
 #pragma unroll
-      for (int wv = 0; wv < kTrainWaves; wv++) {
-        const uint16_t* tw = tiles[wv];
-        a = mfma(ld4(tw + (fD + 16 * mo + lr) * 16 + 4 * lg), ld4(tw + (fH + 16 * mi + lr) * 16 + 4 * lg), a);
+      for (int wv = 0; wv < kTrainWaves; wv += 2) {  // the rows of waves wv and wv + 1 as one 32-wide step
+        const uint16_t *t0 = tiles[wv], *t1 = tiles[wv + 1];
+        const int oD = (fD + 16 * mo + lr) * 16 + 4 * lg, oH = (fH + 16 * mi + lr) * 16 + 4 * lg;
+        a = mfma2(ld4(t0 + oD), ld4(t1 + oD), ld4(t0 + oH), ld4(t1 + oH), a);
       }
       dw[j] = a;
     }
@@ -544,7 +579,7 @@ int guide_reserve(rx_ctx* c, GuideState& G, uint32_t n) {
   E(G.loss_part, (size_t)c->n_cu * 4);
   if (n > G.cap) {
     const size_t N = n;
-    E(G.Z4, N * kGOut * 4);
+    E(G.Z4, N * std::max<size_t>(kGOut * 4, kGHead));  // raw outputs or decoded mixtures
     E(G.pos, N * 16);
     E(G.dir, N * 16);
     E(G.wdir, N * 16);
@@ -727,6 +762,23 @@ int rx_guide_eval(rx_ctx* c, uint32_t n, const float* pos, const float* dir, flo
                       "guide inference"))
     return r;
   if (int r = hip_err(c, hipMemcpyAsync(out48, G.Z4.p, (size_t)n * kGOut * 4, hipMemcpyDeviceToHost, c->stream), "download"))
+    return r;
+  return hip_err(c, hipStreamSynchronize(c->stream), "eval sync");
+}
+
+int rx_guide_eval_mixture(rx_ctx* c, uint32_t n, const float* pos, const float* dir, float* out52) {
+  if (!c || !c->guide) return set_err(c, RX_ERR_STATE, "rx_guide_eval_mixture: call rx_guide_init first");
+  if (n == 0) return RX_OK;
+  if (!pos || !dir || !out52) return set_err(c, RX_ERR_INVALID, "rx_guide_eval_mixture: null array");
+  (void)hipSetDevice(c->device);
+  GuideState& G = *c->guide;
+  if (int r = guide_reserve(c, G, n)) return r;
+  if (int r = upload_queries(c, G, n, pos, dir)) return r;
+  if (int r = hip_err(c, guide_infer(G.net(), G.pos.as<float4>(), G.dir.as<float4>(), 1, G.rows.as<uint32_t>(), n,
+                                     nullptr, G.Z4.as<unsigned char>(), c->n_cu, c->stream),
+                      "guide inference"))
+    return r;
+  if (int r = hip_err(c, hipMemcpyAsync(out52, G.Z4.p, (size_t)n * kGHead, hipMemcpyDeviceToHost, c->stream), "download"))
     return r;
   return hip_err(c, hipStreamSynchronize(c->stream), "eval sync");
 }

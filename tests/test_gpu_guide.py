@@ -57,6 +57,15 @@ class Guide:
                        "eval")
         return out
 
+    def eval_mixture(self, pos, d):
+        out = np.empty((len(pos), 52), np.float32)
+        pos = np.ascontiguousarray(pos, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        self.ctx.check(self.L.rx_guide_eval_mixture(self.ctx.h, len(pos), pos.ctypes.data_as(C.c_void_p),
+                                                    d.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)),
+                       "eval_mixture")
+        return out
+
     def train(self, pos, d, w, pb, wt, want_grad=True):
         arrs = [np.ascontiguousarray(a, np.float32) for a in (pos, d, w, pb, wt)]
         loss = C.c_float()
@@ -102,6 +111,56 @@ def test_forward_matches_oracle(rx, oracle_mod, gpu_ctx):
     print(f"  max |err| {err.max():.4f}, rms(z) {np.sqrt((zr ** 2).mean()):.4f}")
     assert (err <= tol).mean() > 0.999 and err.max() < 0.2 * np.abs(zr).max()
     assert np.abs(z[:, 41:]).max() == 0.0
+
+
+def _decode(z):
+    """The renderer's mixture from raw outputs, in FP64: softmax lobe weights
+    as a cumulative table, unit means, kappa = exp(clamp(z, -3, 8)), the vMF
+    normaliser times the weight, alpha = 0.05 + 0.45 sigmoid(z40)."""
+    z = z.astype(np.float64)
+    lo = z[:, 0:40].reshape(-1, 8, 5)
+    e = np.exp(lo[:, :, 0] - lo[:, :, 0].max(axis=1, keepdims=True))
+    w = e / e.sum(axis=1, keepdims=True)
+    cdf = np.cumsum(w, axis=1)
+    mu = lo[:, :, 1:4] / np.linalg.norm(lo[:, :, 1:4], axis=2, keepdims=True)
+    k = np.exp(np.clip(lo[:, :, 4], -3.0, 8.0))
+    cw = w * k / (2 * np.pi * (1 - np.exp(-2 * k)))
+    alpha = 0.05 + 0.45 / (1 + np.exp(-z[:, 40]))
+    return mu, k, cw, cdf, alpha
+
+
+def test_decoded_mixture_matches_outputs(rx, gpu_ctx):
+    """The decode the renderer samples and evaluates (one 208-byte record per
+    query, written by the inference kernel) against an FP64 decode of the same
+    network's raw outputs, on a row count that is not a multiple of the
+    16-row tile."""
+    print("[guide] decoded mixture", flush=True)
+    rng = np.random.default_rng(5)
+    g = Guide(rx, gpu_ctx)
+    p = _params(rng, scale=0.5)
+    g.set(p)
+    n = 1000
+    pos, d, _, _, _ = _batch(rng, n)
+    z = g.eval(pos, d)
+    m = g.eval_mixture(pos, d)
+    mu, k, cw, cdf, alpha = _decode(z)
+    lobes = m[:, :32].reshape(n, 8, 4)
+    assert np.abs(lobes[:, :, :3] - mu).max() < 2e-6
+    assert np.abs(lobes[:, :, 3] / k - 1).max() < 2e-6
+    assert np.abs(m[:, 40:48] - cdf).max() < 2e-6
+    # cw is the stored table's weights (what the sampler realises) times the vMF normaliser
+    wg = np.diff(m[:, 40:48].astype(np.float64), axis=1, prepend=0.0)
+    expect = wg * k / (2 * np.pi * (1 - np.exp(-2 * k)))
+    big = wg > 1e-6
+    assert np.abs(m[:, 32:40][big] / expect[big] - 1).max() < 1e-5
+    assert np.abs(m[:, 32:40] - expect).max() < 1e-5 * cw.max()
+    assert (m[:, 47] == 1.0).all()  # the sampler's last bin always closes
+    assert np.abs(m[:, 48] - alpha).max() < 1e-6
+    # an index slip in the decode (weight logits read one output late) fails the table check
+    zl = z[:, 1:40:5].astype(np.float64)
+    wrong = np.exp(zl - zl.max(axis=1, keepdims=True))
+    wrong /= wrong.sum(axis=1, keepdims=True)
+    assert np.abs(np.cumsum(wrong, axis=1) - m[:, 40:48]).max() > 1e-2
 
 
 def test_gradient_matches_oracle(rx, oracle_mod, gpu_ctx):
