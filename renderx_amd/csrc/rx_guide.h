@@ -202,10 +202,25 @@ __device__ __forceinline__ void stage_weights(const GuideNet& N, uint16_t* wts) 
   }
 }
 
-// The four layers' biases in LDS, b1 | b2 | b3 | b4 (kBiasCount floats).
+// Output order of the decoding inference: staged output row 16 mt + 4 g + i
+// (the D fragment of output tile mt that lane group g = lane / 16 holds, its
+// slot s = 4 mt + i) is network output s < 10 ? 10 g + s : 40 + 2 g + s - 10.
+// Lane group g then holds lobes 2g and 2g + 1 whole (weight logit, mean,
+// log kappa in slots 0-4 and 5-9), group 0 the selection logit (output 40)
+// in slot 10, and the unused outputs 41-47 fill the remaining slots.
+__device__ __forceinline__ int decode_order(int row) {
+  const int g = (row >> 2) & 3, s = 4 * (row >> 4) + (row & 3);
+  return s < 10 ? 10 * g + s : 40 + 2 * g + s - 10;
+}
+
+// The four layers' biases in LDS, b1 | b2 | b3 | b4 (kBiasCount floats),
+// b4 in decode_order with out_order.
 constexpr int kBiasCount = 3 * kGHid + kGOut;
-__device__ __forceinline__ void stage_bias(const GuideNet& N, float* bias) {
-  for (int e = threadIdx.x; e < kBiasCount; e += blockDim.x) bias[e] = N.b[e / kGHid][e % kGHid];
+__device__ __forceinline__ void stage_bias(const GuideNet& N, float* bias, bool out_order) {
+  for (int e = threadIdx.x; e < kBiasCount; e += blockDim.x) {
+    const int L = e / kGHid, o = e % kGHid;
+    bias[e] = N.b[L][L == 3 && out_order ? decode_order(o) : o];
+  }
 }
 
 // Inference order of the encoding (k_guide_fwd): at k-step ks the lanes of
@@ -219,8 +234,9 @@ __device__ __forceinline__ int enc_chain_pair(int lg, int m) {
   return lg < 3 ? 3 * m + lg : 36 + 3 * (m & 3) + (m >> 2);
 }
 // stage_weights with the first layer's input columns in the chain order:
-// staged column 16 ks + 4 lg + 2 u + v holds feature 2 enc_chain_pair(lg, 2 ks + u) + v.
-__device__ __forceinline__ void stage_weights_chain(const GuideNet& N, uint16_t* wts) {
+// staged column 16 ks + 4 lg + 2 u + v holds feature 2 enc_chain_pair(lg, 2 ks + u) + v;
+// with out_order the last layer's rows are in decode_order.
+__device__ __forceinline__ void stage_weights_chain(const GuideNet& N, uint16_t* wts, bool out_order) {
   for (int e = threadIdx.x; e < kGHid * kGIn; e += blockDim.x) {
     const int r = e / kGIn, c = e % kGIn;
     const int ks = c >> 4, lg = (c >> 2) & 3, u = (c >> 1) & 1, v = c & 1;
@@ -230,8 +246,9 @@ __device__ __forceinline__ void stage_weights_chain(const GuideNet& N, uint16_t*
     const int K = kLayerIn[L], NO = kLayerOut[L];
     for (int e = threadIdx.x; e < NO * K / 4; e += blockDim.x) {  // 4 bf16 per step
       const int r = (4 * e) / K, col = (4 * e) % K;
+      const int src = L == 3 && out_order ? decode_order(r) : r;
       *reinterpret_cast<uint2*>(&wts[kWOff[L] + r * (K + kWPad) + col]) =
-          *reinterpret_cast<const uint2*>(N.Wt[L] + (size_t)r * K + col);
+          *reinterpret_cast<const uint2*>(N.Wt[L] + (size_t)src * K + col);
     }
   }
 }

@@ -29,11 +29,12 @@ namespace rx {
 // lane computes the encoding features of its own B fragments (row l%16,
 // features 16 ks + 4(l/16) .. +3).  Outputs: the raw fp32 outputs (Z4, 48 per
 // row) or, with DECODE, the decoded mixture as kGHead bytes per row (the
-// renderer's form, GuideHead), staged through LDS so that rows are written
-// whole.  Query row r is pos[r * stride], dir[r * stride].
-// 16 waves share one copy of the weights: one block per CU (38.6 KB of
-// weights and 16 x 3 KB of output staging in LDS), 4 waves per SIMD
-constexpr int kFwdBlock = 1024;
+// renderer's form, GuideHead), decoded in the registers of the last layer's
+// D fragments with its outputs staged in decode_order.  Query row r is
+// pos[r * stride], dir[r * stride].
+// 16 waves share one copy of the weights (39.6 KB of LDS with the biases);
+// two blocks per CU, 8 waves per SIMD at 64 VGPRs
+constexpr int kFwdBlock = 1024, kFwdBlocksPerCu = 2;
 constexpr int kFwdWaves = kFwdBlock / 64;
 
 // One layer: acc[mt] = W[16 mt .. , :] * in + bias for the layer's MT output
@@ -74,12 +75,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <bool DECODE>
-__global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float4* pos, const float4* dir,
+__global__ void __launch_bounds__(kFwdBlock, kFwdBlocksPerCu) k_guide_fwd(GuideNet N, const float4* pos, const float4* dir,
                                                          uint32_t stride, const uint32_t* rows, uint32_t cap,
                                                          float* Z4, unsigned char* head) {
   __shared__ __attribute__((aligned(16))) uint16_t wts[kWLds];
   __shared__ __attribute__((aligned(16))) float bias[kBiasCount];
-  __shared__ __attribute__((aligned(16))) float zt[DECODE ? kFwdWaves : 1][16 * kGOut];  // a tile's outputs
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
   const uint32_t M = min(*rows, cap);
   const uint32_t nTiles = (M + 15) / 16;
@@ -96,10 +96,13 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
   uint32_t tile = blockIdx.x * kFwdWaves + wave;
   auto query_row = [&](uint32_t t) { return (size_t)min(min(t, nTiles - 1) * 16 + (uint32_t)lr, M - 1) * stride; };
   float4 p4n = pos[query_row(tile)], d4n = dir[query_row(tile)];
-  stage_weights_chain(N, wts);
-  stage_bias(N, bias);
+  stage_weights_chain(N, wts, DECODE);
+  stage_bias(N, bias, DECODE);
   __syncthreads();
   for (; tile < nTiles; tile += tstep) {
+    // the weight fragments are read from LDS for every tile: without a fence
+    // in the loop the compiler hoists all 34 of them (136 VGPRs) and spills
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint32_t r0 = tile * 16;
     const float4 p4 = p4n, d4 = d4n;
     p4n = pos[query_row(tile + tstep)];
@@ -127,7 +130,7 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
         }
         pr[u] = bf2_pack(sn, cs);
       }
-      in[ks] = v4s{(short)(pr[0] & 0xffffu), (short)(pr[0] >> 16), (short)(pr[1] & 0xffffu), (short)(pr[1] >> 16)};
+      in[ks] = __builtin_bit_cast(v4s, make_uint2(pr[0], pr[1]));
     }
     v4f acc[4];
     v4s h[4];
@@ -149,61 +152,55 @@ __global__ void __launch_bounds__(kFwdBlock) k_guide_fwd(GuideNet N, const float
             make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
       continue;
     }
-    float* z = zt[DECODE ? wave : 0];
-#pragma unroll
-    for (int mt = 0; mt < 3; mt++)
-      *reinterpret_cast<float4*>(z + lr * kGOut + mt * 16 + 4 * lg) =
-          make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
-    wave_lds_sync();
-    {  // four lanes per row, two lobes each, written to the row's decoded record directly
-      const int r = lane >> 2, sub = lane & 3;
-      const float* zr = z + r * kGOut;
-      unsigned char* hrow = head + (size_t)min(r0 + (uint32_t)r, M - 1) * kGHead;
-      const int j0 = 2 * sub, j1 = 2 * sub + 1;
-      const float l0 = zr[5 * j0], l1 = zr[5 * j1];
+    {  // decode in the D fragments (decode_order): lane group lg holds lobes 2 lg and 2 lg + 1 of row lr
+      unsigned char* hrow = head + (size_t)min(r0 + (uint32_t)lr, M - 1) * kGHead;
+      const float l0 = acc[0][0], l1 = acc[1][1];
       float mx = fmaxf(l0, l1);
-      mx = fmaxf(mx, __shfl_xor(mx, 1));
-      mx = fmaxf(mx, __shfl_xor(mx, 2));
+      mx = fmaxf(mx, __shfl_xor(mx, 16));
+      mx = fmaxf(mx, __shfl_xor(mx, 32));
       const float e0 = __expf(l0 - mx), e1 = __expf(l1 - mx);
-      // cumulative sums over the row's lanes in lobe order (exclusive prefix of the pair sums)
+      // cumulative sums over the row's lane groups in lobe order (exclusive prefix of the pair sums)
       const float pair = e0 + e1;
-      const float p1 = __shfl_xor(pair, 1), p2 = __shfl_xor(pair, 2), p3 = __shfl_xor(pair, 3);
+      const float p1 = __shfl_xor(pair, 16), p2 = __shfl_xor(pair, 32), p3 = __shfl_xor(pair, 48);
       const float total = pair + p1 + p2 + p3;
-      // lanes sub = 0..3 hold pairs 0..3; the sum of the pairs before this lane's
-      const float before = (sub == 0) ? 0.0f : (sub == 1) ? p1 : (sub == 2) ? (p2 + p3) : (p1 + p2 + p3);
+      // groups lg = 0..3 hold pairs 0..3; the sum of the pairs before this group's
+      const float before = (lg == 0) ? 0.0f : (lg == 1) ? p1 : (lg == 2) ? (p2 + p3) : (p1 + p2 + p3);
       // approximate reciprocals (1 ulp) in the decode: the sampler and the
       // density both read the stored values, so they stay consistent
       const float inv = __builtin_amdgcn_rcpf(total);
       const float cdf0 = (before + e0) * inv;
-      const float cdf1 = sub == 3 ? 1.0f : (before + e0 + e1) * inv;
+      const float cdf1 = lg == 3 ? 1.0f : (before + e0 + e1) * inv;
       const float cprev = before * inv;
       const float wts2[2] = {cdf0 - cprev, cdf1 - cdf0};
       const float cdfs[2] = {cdf0, cdf1};
+      // lobe 2 lg: mean in slots 1-3, log kappa in slot 4; lobe 2 lg + 1: slots 6-8 and 9
+      const float mu[2][3] = {{acc[0][1], acc[0][2], acc[0][3]}, {acc[1][2], acc[1][3], acc[2][0]}};
+      const float lk[2] = {acc[1][0], acc[2][1]};
 #pragma unroll
       for (int u = 0; u < 2; u++) {
-        const int j = u ? j1 : j0;
-        const float m0 = zr[5 * j + 1], m1 = zr[5 * j + 2], m2 = zr[5 * j + 3];
+        const int j = 2 * lg + u;
+        const float m0 = mu[u][0], m1 = mu[u][1], m2 = mu[u][2];
         const float il = __builtin_amdgcn_rsqf(fmaxf(m0 * m0 + m1 * m1 + m2 * m2, 1e-12f));
-        const float k = __expf(fminf(8.0f, fmaxf(-3.0f, zr[5 * j + 4])));
+        const float k = __expf(fminf(8.0f, fmaxf(-3.0f, lk[u])));
         reinterpret_cast<float4*>(hrow)[j] = make_float4(m0 * il, m1 * il, m2 * il, k);
         reinterpret_cast<float*>(hrow + kGHeadCw)[j] =
             wts2[u] * k * __builtin_amdgcn_rcpf(6.28318530717958647692f * (1.0f - __expf(-2.0f * k)));
         reinterpret_cast<float*>(hrow + kGHeadCdf)[j] = cdfs[u];
       }
-      // the row's four lanes store the same selection record
+      // the selection logit (slot 10 of group 0) to the row's four lanes, which store the same record
+      const float al = __shfl(acc[2][2], lr);
       *reinterpret_cast<float4*>(hrow + kGHeadAlpha) =
-          make_float4(kAlphaMin + kAlphaSpan / (1.0f + __expf(-zr[40])), 0.0f, 0.0f, 0.0f);
+          make_float4(kAlphaMin + kAlphaSpan / (1.0f + __expf(-al)), 0.0f, 0.0f, 0.0f);
     }
-    wave_lds_sync();  // the tile's outputs are read before the next tile overwrites them
   }
 }
 
 hipError_t guide_infer(const GuideNet& N, const float4* pos, const float4* dir, uint32_t stride, const uint32_t* rows,
                        uint32_t cap, float* Z4, unsigned char* head, int n_cu, hipStream_t s) {
-  // one block per CU (the LDS of one fits), each grid-striding over the tiles
+  // as many blocks as are resident at once, each grid-striding over the tiles
   const uint32_t tiles = (cap + 15) / 16;
   const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves,
-                                                                   (uint32_t)n_cu));
+                                                                   (uint32_t)n_cu * kFwdBlocksPerCu));
   if (head)
     hipLaunchKernelGGL(k_guide_fwd<true>, dim3(blocks), dim3(kFwdBlock), 0, s, N, pos, dir, stride, rows, cap, Z4,
                        head);
@@ -307,7 +304,7 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_train(GuideNet N, TrainRo
   const uint32_t M = min(*rows, cap);
   const uint32_t nIter = (M + 16 * kTrainWaves - 1) / (16 * kTrainWaves);
   stage_weights(N, wts);
-  stage_bias(N, bias);
+  stage_bias(N, bias, false);
   for (int L = 1; L < 4; L++) {  // backward weights, [in][out]
     const int K = kLayerOut[L], ld = K + kRPad;
     for (int e = threadIdx.x; e < kGHid * K / 4; e += blockDim.x) {
