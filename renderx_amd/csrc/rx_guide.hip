@@ -258,7 +258,7 @@ __device__ __forceinline__ void param_loc(int idx, int& L, int& off, bool& isb) 
 // waves split the 68 16x16 tiles of the weight gradients, each contracting
 // over the block's 64 rows, and 240 threads sum the bias gradients.  Every
 // block writes its own partial (kGParams floats) and loss sum; k_adam_reduce
-// adds the partials in block order, so the step is deterministic.
+// adds the partials in a fixed order, so the step is deterministic.
 constexpr int kTrainWaves = 4;
 constexpr int kTrainBlock = 64 * kTrainWaves;
 // stored tile rows (features) of one wave: X 96, H1..H3, D1..D3 64 each, dZ 48
@@ -509,18 +509,32 @@ __global__ void __launch_bounds__(kTrainBlock) k_guide_train(GuideNet N, TrainRo
   if (threadIdx.x == 0) loss_part[blockIdx.x] = lsum[0] + lsum[1] + lsum[2] + lsum[3];
 }
 
-// Mean gradient of a step from the blocks' partials (added in block order),
-// then the Adam update of the FP32 masters and their BF16 copies.
-__global__ void k_adam_reduce(AdamArgs A, const float* partial, uint32_t nblocks) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= kGParams) return;
+// Mean gradient of a step from the training blocks' partials, then the Adam
+// update of the FP32 masters and their BF16 copies.  A block takes 32
+// parameters; its 8 lane groups of 32 add the partials b = g, g + 8, ... in
+// ascending order, and the 8 group sums are added in group order: a fixed
+// order, so the step is deterministic, with 8 loads in flight per parameter
+// where one thread per parameter walked all partials.
+constexpr int kAdamParams = 32, kAdamGroups = 8;
+__global__ void __launch_bounds__(kAdamParams * kAdamGroups) k_adam_reduce(AdamArgs A, const float* partial,
+                                                                           uint32_t nblocks) {
+  __shared__ float red[kAdamGroups][kAdamParams];
+  const int pi = threadIdx.x % kAdamParams, grp = threadIdx.x / kAdamParams;
+  const int idx = blockIdx.x * kAdamParams + pi;
+  float gs = 0.0f;
+  if (idx < kGParams)
+    for (uint32_t b = grp; b < nblocks; b += kAdamGroups) gs += partial[(size_t)b * kGParams + idx];
+  red[grp][pi] = gs;
+  __syncthreads();
+  if (grp != 0 || idx >= kGParams) return;
+  float g = red[0][pi];
+#pragma unroll
+  for (int k = 1; k < kAdamGroups; k++) g += red[k][pi];
   int L, off;
   bool isb;
   param_loc(idx, L, off, isb);
   const int IN = kLayerIn[L], OUT = kLayerOut[L];
   const uint32_t M = min(*A.rows, A.cap);
-  float g = 0.0f;
-  for (uint32_t b = 0; b < nblocks; b++) g += partial[(size_t)b * kGParams + idx];
   g = M ? g / (float)M : 0.0f;
   if (A.grad_out) A.grad_out[idx] = g;
   float m = A.b1 * A.m[idx] + (1.0f - A.b1) * g;
@@ -625,7 +639,8 @@ static int guide_train_device(rx_ctx* c, GuideState& G, uint32_t cap, float* gra
   A.eps = G.prm.eps;
   A.c1 = 1.0f - powf(A.b1, (float)G.step);
   A.c2 = 1.0f - powf(A.b2, (float)G.step);
-  hipLaunchKernelGGL(k_adam_reduce, dim3((kGParams + 255) / 256), dim3(256), 0, s, A, G.tpart.as<float>(), blocks);
+  hipLaunchKernelGGL(k_adam_reduce, dim3((kGParams + kAdamParams - 1) / kAdamParams), dim3(kAdamParams * kAdamGroups),
+                     0, s, A, G.tpart.as<float>(), blocks);
   return hip_err(c, hipGetLastError(), "guide train launch");
 }
 
