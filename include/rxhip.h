@@ -103,7 +103,7 @@ enum rx_render_flags {
  * order and the random numbers are keyed by pixel and sample); only the time
  * changes.  Zero fields take the defaults. */
 typedef struct rx_render_tuning {
-  uint32_t path_slots;    /* path states resident at once (default 2^25): a pass renders
+  uint32_t path_slots;    /* path states resident at once (default 2^26): a pass renders
                              min(spp, path_slots / pixels) samples of every pixel (>= 1) */
   uint32_t blocks_per_cu; /* resident 128-thread blocks per CU of the render kernels
                              (default and maximum: the traversal kernel's occupancy limit) */

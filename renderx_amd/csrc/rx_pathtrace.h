@@ -44,7 +44,7 @@ constexpr int kCntWords = kSpareWork + (int)kWorkWords;
 static_assert(kShadowBase + kMaxBounce <= kSpareWords && kSpareWords + 64 <= kGuideRowBase &&
                   kGuideRowBase + kMaxBounce <= kWorkBase,
               "counter regions do not overlap");
-constexpr uint32_t kDefaultPathSlots = 1u << 25;  // rx_render_tuning.path_slots default (~5.7 GB of path state)
+constexpr uint32_t kDefaultPathSlots = 1u << 26;  // rx_render_tuning.path_slots default (12.6 GB of path state)
 constexpr int kMaxCross = 4;          // index-matched boundary crossings per shadow ray
 constexpr uint32_t FLAG_MIS = 1u;     // last scatter was RANDOM (only transparent crossings since)
 // Guided mode: bits 8.. of the flags count the path's guided vertices so far.
