@@ -119,6 +119,10 @@ typedef struct rx_render_tuning {
                              wave holds the samples of two pixels), lanes for batches (a wave holds 64
                              neighbouring pixels of a scene, too far apart to share a walk).  The hits are
                              identical (the persistent batch launch traces them per lane) */
+  uint32_t tail_bounce;   /* rx_render in RX_MODE_PT on a scene without media: the first bounce whose paths
+                             one launch carries to their end (each lane traces, shades and traces again)
+                             instead of one extend and one shade launch per bounce; 0 the default (3),
+                             a bounce above max_depth none */
 } rx_render_tuning;
 
 typedef struct rx_render_stats {

@@ -27,17 +27,18 @@ RENDER_STATS = np.dtype([("rays", "<u8"), ("extend_rays", "<u8"), ("shadow_rays"
                          ("extend_launches", "<u8"), ("guide_rows", "<u8"), ("guide_train_rows", "<u8")])
 RENDER_TIMING, RENDER_COUNT_WORK = 1, 2
 RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use_graph", "<u4"), ("stack_depth", "<u4"),
-                          ("block_size", "<u4"), ("camera_rays", "<u4")])
+                          ("block_size", "<u4"), ("camera_rays", "<u4"), ("tail_bounce", "<u4")])
 GUIDE_PARAMS = np.dtype([("train_passes", "<u4"), ("batch_size", "<u4"), ("steps_per_pass", "<u4"), ("learning_rate", "<f4"),
                          ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("seed", "<u4"), ("flags", "<u4"),
                          ("samples_per_pass", "<u4")])
 MODE_PT, MODE_NEE, MODE_GUIDED, MODE_BPT = 0, 1, 2, 3
 
 
-def _tuning(path_slots, blocks_per_cu, use_graph, stack_depth, block_size, camera_rays=0):
+def _tuning(path_slots, blocks_per_cu, use_graph, stack_depth, block_size, camera_rays=0, tail_bounce=0):
     t = np.zeros(1, RENDER_TUNING)
     t["path_slots"], t["blocks_per_cu"], t["use_graph"] = path_slots, blocks_per_cu, use_graph
     t["stack_depth"], t["block_size"], t["camera_rays"] = stack_depth, block_size, camera_rays
+    t["tail_bounce"] = tail_bounce
     return t
 GUIDE_LR_DECAY = 1
 GUIDE_KEEP_SAMPLES = 2
@@ -172,10 +173,14 @@ class PathTracerHIP:
         self.ctx.check(lib().rx_guide_get_weights(self.ctx.h, ptr(w)), "rx_guide_get_weights")
         return w
 
-    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0, stack_depth=0, block_size=0, camera_rays=0):
-        """Launch shape of renderPixels (rx_render_tuning); images do not depend on it."""
+    def set_tuning(self, path_slots=0, blocks_per_cu=0, use_graph=0, stack_depth=0, block_size=0, camera_rays=0,
+                   tail_bounce=0):
+        """Launch shape of renderPixels (rx_render_tuning); images do not depend on it.
+        tail_bounce: first bounce of the one-launch tail of RX_MODE_PT (0 the
+        default, above max_depth none)."""
         self.ctx.check(lib().rx_set_render_tuning(self.ctx.h, ptr(_tuning(path_slots, blocks_per_cu, use_graph,
-                                                                          stack_depth, block_size, camera_rays))),
+                                                                          stack_depth, block_size, camera_rays,
+                                                                          tail_bounce))),
                        "rx_set_render_tuning")
 
     def get_tuning(self):

@@ -1093,6 +1093,163 @@ struct ShadeLds {
   uint2 stack[kLdsDepth * kRBlock];
   uint32_t cont_list[kStageCap], nee_list[MODE >= 1 ? kStageCap : 1], counts[4];
 };
+// Bounce b of path p (queue entry i of the bounce): the reference's vertex
+// (MCRenderer::samplePathIter :169-201 with PathTracer's accumulation), its
+// next-event query (MODE >= 1) and the continuation written back to the path
+// state; cont: the path goes on to bounce b + 1, nee: a shadow query was
+// written.  shade_body runs it for a bounce's queue, k_tail for the last
+// bounces of a path in one lane.
+template <int MODE, bool BATCH, bool VOL>
+__device__ __forceinline__ void shade_path(const DevScene& S0, const PathBufs& B, const PassArgs& P, uint32_t b,
+                                           const GuideArgs& GA, const BatchArgs& BA, uint32_t p, uint32_t i,
+                                           uint2* lds, uint32_t& rays, bool& cont, bool& nee) {
+  DevScene S = S0;
+  const PathCtx pc = path_ctx<BATCH>(P, BA, p, S);
+  const uint32_t pix = pc.pix, smp = pc.smp;
+  const bool russian = b >= pc.rr_start;
+  const float4 o4 = B.o[p], d4 = B.d[p], be = B.beta[p];
+  const int4 s = B.s[p];
+  const int2 h = B.hit[p];
+  const Vec3 o = xyz(o4), d = xyz(d4);
+  Vec3 beta = xyz(be);
+  const int inside = s.x;
+  const float t = __int_as_float(h.x);
+  const int pos = h.y;
+  U4 r0 = philox(U4{smp, b, 0u, 0u}, pix, pc.key1);
+  U4 r1 = philox(U4{smp, b, 1u, 0u}, pix, pc.key1);
+  const float u[8] = {u01(r0.x), u01(r0.y), u01(r0.z), u01(r0.w), u01(r1.x), u01(r1.y), u01(r1.z), u01(r1.w)};
+  int hitObj = NONE;
+  Vec3 hn = v3(0, 0, 0);
+  if (pos >= 0) {
+    hitObj = __float_as_int(S.tree.tv1[pos].w);
+    hn = xyz(S.tree.tn[pos]);
+  }
+  const int obj = inside != NONE ? inside : hitObj;  // MCRenderer::samplePathIter :169-181
+  if (obj != NONE) {
+    Scat sc;
+    GuideHead head;
+    float originScale;
+    scatter_at<VOL, MODE == 2>(S, o, d, t, pos, hitObj, hn, inside, russian, u, lds, B, rays, false, GA, i, smp, b,
+                               pix, pc.key1, sc, head, originScale);
+    sc.originProb *= originScale;
+    // segment into the new vertex: decay by the incoming ray's medium, 1/originProb
+    const float seg = length(sc.x - o);
+    beta = beta * decay(S, inside, seg);
+    // vec3 / 0 leaves the vector unchanged in the reference (nvVector.h operator/=)
+    if (sc.originProb != 0.0f) beta = beta * (1.0f / sc.originProb);
+    if (MODE >= 1 && sc.neeKind && b < pc.max_depth) {
+      // next-event estimation to an area light: one light point, MIS with the continuation density.
+      // The light point is the path's vertex b + 1 (+ the boundary crossings of the shadow
+      // ray), so like a continuation it needs a bounce left.
+      U4 r2 = philox(U4{smp, b, 2u, 0u}, pix, pc.key1);
+      float us = u01(r2.x), u1 = u01(r2.y), u2 = u01(r2.z);
+      int lo = 0, hi = S.n_etri - 1;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (us < S.ecdf[mid]) hi = mid;
+        else lo = mid + 1;
+      }
+      const float4 e0 = S.etri[3 * lo], e1 = S.etri[3 * lo + 1], e2 = S.etri[3 * lo + 2];
+      const Vec3 nL = xyz(S.enrm[lo]);
+      float su = sqrtf(u1), bu = 1.0f - su, bv = u2 * su;
+      Vec3 y = xyz(e0) * bu + xyz(e1) * bv + xyz(e2) * (1.0f - bu - bv);
+      Vec3 wv = y - sc.x;
+      float r2l = dot(wv, wv), rl = sqrtf(r2l);
+      Vec3 w = wv * (1.0f / rl);
+      float cosL = -dot(nL, w);
+      if (cosL > 0 && rl > 0) {
+        const DevObject& LO = S.objs[__float_as_int(e0.w)];
+        float pL = r2l / (cosL * S.total_area);
+        Vec3 f;
+        float pB, cx = 1.0f;
+        if (sc.neeKind == 1 && sc.guided) {
+          float pbl;
+          bsdf_nrr(S.objs[sc.neeObj], hn, d, w, f, pbl);
+          pB = sc.qrr * (head.alpha * guide_pdf(head, w) + (1.0f - head.alpha) * pbl);
+          cx = fmaxf(fabsf(dot(hn, w)), 1e-6f);
+        } else if (sc.neeKind == 1) {
+          surface_bsdf(S.objs[sc.neeObj], hn, d, w, russian, f, pB);
+          cx = fmaxf(fabsf(dot(hn, w)), 1e-6f);
+        } else {
+          const DevObject& M = S.objs[sc.neeObj];
+          float ph = hg(M.g, dot(d, w));
+          f = ld3(M.c1) * ph;
+          pB = (russian ? M.albedo : 1.0f) *
+               (sc.guided ? head.alpha * guide_pdf(head, w) + (1.0f - head.alpha) * ph : ph);
+        }
+        Vec3 c = beta * f * (cx / (pL + pB)) * ld3(LO.c1);
+        if (maxc(c) > 0) {
+          nee = true;
+          B.so[p] = f4(sc.x, rl);
+          B.sd[p] = f4(w, 0.0f);
+          B.sc[p] = f4(c, 0.0f);
+          B.ss[p] = sc.neeKind == 1 ? make_int2(sc.inside, sc.neeObj) : make_int2(sc.neeObj, NONE);
+        }
+      }
+    }
+    if (sc.emitter) {
+      Vec3 le = sc.dirProb != 0.0f ? sc.color * (1.0f / sc.dirProb) : sc.color;
+      float w = 1.0f;
+      if (MODE >= 1 && (s.w & FLAG_MIS)) {
+        Vec3 lp = xyz(B.lastp[p]);
+        Vec3 dv = sc.x - lp;
+        float cosL = fabsf(dot(hn, d));
+        float pL = dot(dv, dv) / (fmaxf(cosL, 1e-12f) * S.total_area);
+        w = be.w / (be.w + pL);
+      }
+      Vec3 add = beta * le * w;
+      float4 L = B.L[p];
+      B.L[p] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.0f);
+    } else if (length(sc.dir) >= 0.5f) {
+      bool alive = true;
+      Vec3 nc = v3(0, 0, 0);
+      if (sc.contact != NONE && sc.contactPos >= 0) nc = xyz(S.tree.tn[sc.contactPos]);
+      if (sc.inside == NONE && sc.contact != NONE) {  // shading-normal light-leak guard (:183-201)
+        float a = dot(nc, -d), c2 = dot(nc, sc.dir);
+        if (a * a <= 0 || c2 * c2 <= 0) alive = false;
+      }
+      if (alive && b + 1 <= pc.max_depth) {
+        float cosT = 1.0f;
+        if (sc.contact != NONE && S.objs[sc.contact].type != OBJ_VPM)
+          cosT = fmaxf(fabsf(dot(nc, sc.dir)), 1e-6f);
+        const Vec3 beta_arr = beta;  // throughput arriving at the vertex
+        beta = beta * sc.color * (sc.dirProb != 0.0f ? cosT / sc.dirProb : cosT);
+        uint32_t flags = (uint32_t)s.w;
+        float lastPdf = be.w;
+        if (sc.random) {
+          flags = FLAG_MIS;
+          if (sc.guided) {
+            lastPdf = sc.pdfB;
+            if (GA.tag > 0.0f) {  // training record of this vertex
+              float* r = GA.rec + ((size_t)p * GA.R + b) * kRecFloats;
+              const float4 Lp = B.L[p];
+              const float vals[kRecFloats] = {sc.x.x, sc.x.y, sc.x.z, -d.x, -d.y, -d.z, sc.dir.x, sc.dir.y,
+                                              sc.dir.z, sc.pmix, sc.pb, lum(v3(Lp.x, Lp.y, Lp.z)), lum(beta_arr),
+                                              sc.qrr, 0.0f, GA.tag};
+              for (int k = 0; k < kRecFloats; k++) r[k] = vals[k];
+            }
+          } else if (sc.neeKind == 1) {
+            Vec3 fdummy;
+            surface_bsdf(S.objs[sc.neeObj], hn, d, sc.dir, russian, fdummy, lastPdf);
+          } else {
+            lastPdf = sc.dirProb;
+          }
+          B.lastp[p] = f4(sc.x, 0.0f);
+        } else if (!sc.transparent) {
+          flags = 0;
+        }
+        const uint32_t nguided = ((uint32_t)s.w >> kGuidedShift) + (sc.guided ? 1u : 0u);
+        flags = (flags & ((1u << kGuidedShift) - 1u)) | (nguided << kGuidedShift);
+        B.o[p] = f4(sc.x, 0.0f);
+        B.d[p] = f4(sc.dir, 0.0f);
+        B.s[p] = make_int4(sc.inside, sc.contact, sc.contactPos, (int)flags);
+        B.beta[p] = f4(beta, lastPdf);
+        cont = true;
+      }
+    }
+  }
+}
+
 template <int MODE, bool BATCH, bool VOL>
 __device__ __forceinline__ void shade_body(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA, BatchArgs BA,
                                            ShadeLds<MODE>& sh) {
@@ -1117,152 +1274,8 @@ __device__ __forceinline__ void shade_body(DevScene S0, PathBufs B, PassArgs P, 
     uint32_t p = 0;
     if (i < n) {
       p = qin[i];
-      DevScene S = S0;
-      const PathCtx pc = path_ctx<BATCH>(P, BA, p, S);
-      const uint32_t pix = pc.pix, smp = pc.smp;
-      const bool russian = b >= pc.rr_start;
       segs++;
-      const float4 o4 = B.o[p], d4 = B.d[p], be = B.beta[p];
-      const int4 s = B.s[p];
-      const int2 h = B.hit[p];
-      const Vec3 o = xyz(o4), d = xyz(d4);
-      Vec3 beta = xyz(be);
-      const int inside = s.x;
-      const float t = __int_as_float(h.x);
-      const int pos = h.y;
-      U4 r0 = philox(U4{smp, b, 0u, 0u}, pix, pc.key1);
-      U4 r1 = philox(U4{smp, b, 1u, 0u}, pix, pc.key1);
-      const float u[8] = {u01(r0.x), u01(r0.y), u01(r0.z), u01(r0.w), u01(r1.x), u01(r1.y), u01(r1.z), u01(r1.w)};
-      int hitObj = NONE;
-      Vec3 hn = v3(0, 0, 0);
-      if (pos >= 0) {
-        hitObj = __float_as_int(S.tree.tv1[pos].w);
-        hn = xyz(S.tree.tn[pos]);
-      }
-      const int obj = inside != NONE ? inside : hitObj;  // MCRenderer::samplePathIter :169-181
-      if (obj != NONE) {
-        Scat sc;
-        GuideHead head;
-        float originScale;
-        scatter_at<VOL, MODE == 2>(S, o, d, t, pos, hitObj, hn, inside, russian, u, lds, B, rays, false, GA, i, smp, b,
-                                   pix, pc.key1, sc, head, originScale);
-        sc.originProb *= originScale;
-        // segment into the new vertex: decay by the incoming ray's medium, 1/originProb
-        const float seg = length(sc.x - o);
-        beta = beta * decay(S, inside, seg);
-        // vec3 / 0 leaves the vector unchanged in the reference (nvVector.h operator/=)
-        if (sc.originProb != 0.0f) beta = beta * (1.0f / sc.originProb);
-        if (MODE >= 1 && sc.neeKind && b < pc.max_depth) {
-          // next-event estimation to an area light: one light point, MIS with the continuation density.
-          // The light point is the path's vertex b + 1 (+ the boundary crossings of the shadow
-          // ray), so like a continuation it needs a bounce left.
-          U4 r2 = philox(U4{smp, b, 2u, 0u}, pix, pc.key1);
-          float us = u01(r2.x), u1 = u01(r2.y), u2 = u01(r2.z);
-          int lo = 0, hi = S.n_etri - 1;
-          while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (us < S.ecdf[mid]) hi = mid;
-            else lo = mid + 1;
-          }
-          const float4 e0 = S.etri[3 * lo], e1 = S.etri[3 * lo + 1], e2 = S.etri[3 * lo + 2];
-          const Vec3 nL = xyz(S.enrm[lo]);
-          float su = sqrtf(u1), bu = 1.0f - su, bv = u2 * su;
-          Vec3 y = xyz(e0) * bu + xyz(e1) * bv + xyz(e2) * (1.0f - bu - bv);
-          Vec3 wv = y - sc.x;
-          float r2l = dot(wv, wv), rl = sqrtf(r2l);
-          Vec3 w = wv * (1.0f / rl);
-          float cosL = -dot(nL, w);
-          if (cosL > 0 && rl > 0) {
-            const DevObject& LO = S.objs[__float_as_int(e0.w)];
-            float pL = r2l / (cosL * S.total_area);
-            Vec3 f;
-            float pB, cx = 1.0f;
-            if (sc.neeKind == 1 && sc.guided) {
-              float pbl;
-              bsdf_nrr(S.objs[sc.neeObj], hn, d, w, f, pbl);
-              pB = sc.qrr * (head.alpha * guide_pdf(head, w) + (1.0f - head.alpha) * pbl);
-              cx = fmaxf(fabsf(dot(hn, w)), 1e-6f);
-            } else if (sc.neeKind == 1) {
-              surface_bsdf(S.objs[sc.neeObj], hn, d, w, russian, f, pB);
-              cx = fmaxf(fabsf(dot(hn, w)), 1e-6f);
-            } else {
-              const DevObject& M = S.objs[sc.neeObj];
-              float ph = hg(M.g, dot(d, w));
-              f = ld3(M.c1) * ph;
-              pB = (russian ? M.albedo : 1.0f) *
-                   (sc.guided ? head.alpha * guide_pdf(head, w) + (1.0f - head.alpha) * ph : ph);
-            }
-            Vec3 c = beta * f * (cx / (pL + pB)) * ld3(LO.c1);
-            if (maxc(c) > 0) {
-              nee = true;
-              B.so[p] = f4(sc.x, rl);
-              B.sd[p] = f4(w, 0.0f);
-              B.sc[p] = f4(c, 0.0f);
-              B.ss[p] = sc.neeKind == 1 ? make_int2(sc.inside, sc.neeObj) : make_int2(sc.neeObj, NONE);
-            }
-          }
-        }
-        if (sc.emitter) {
-          Vec3 le = sc.dirProb != 0.0f ? sc.color * (1.0f / sc.dirProb) : sc.color;
-          float w = 1.0f;
-          if (MODE >= 1 && (s.w & FLAG_MIS)) {
-            Vec3 lp = xyz(B.lastp[p]);
-            Vec3 dv = sc.x - lp;
-            float cosL = fabsf(dot(hn, d));
-            float pL = dot(dv, dv) / (fmaxf(cosL, 1e-12f) * S.total_area);
-            w = be.w / (be.w + pL);
-          }
-          Vec3 add = beta * le * w;
-          float4 L = B.L[p];
-          B.L[p] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.0f);
-        } else if (length(sc.dir) >= 0.5f) {
-          bool alive = true;
-          Vec3 nc = v3(0, 0, 0);
-          if (sc.contact != NONE && sc.contactPos >= 0) nc = xyz(S.tree.tn[sc.contactPos]);
-          if (sc.inside == NONE && sc.contact != NONE) {  // shading-normal light-leak guard (:183-201)
-            float a = dot(nc, -d), c2 = dot(nc, sc.dir);
-            if (a * a <= 0 || c2 * c2 <= 0) alive = false;
-          }
-          if (alive && b + 1 <= pc.max_depth) {
-            float cosT = 1.0f;
-            if (sc.contact != NONE && S.objs[sc.contact].type != OBJ_VPM)
-              cosT = fmaxf(fabsf(dot(nc, sc.dir)), 1e-6f);
-            const Vec3 beta_arr = beta;  // throughput arriving at the vertex
-            beta = beta * sc.color * (sc.dirProb != 0.0f ? cosT / sc.dirProb : cosT);
-            uint32_t flags = (uint32_t)s.w;
-            float lastPdf = be.w;
-            if (sc.random) {
-              flags = FLAG_MIS;
-              if (sc.guided) {
-                lastPdf = sc.pdfB;
-                if (GA.tag > 0.0f) {  // training record of this vertex
-                  float* r = GA.rec + ((size_t)p * GA.R + b) * kRecFloats;
-                  const float4 Lp = B.L[p];
-                  const float vals[kRecFloats] = {sc.x.x, sc.x.y, sc.x.z, -d.x, -d.y, -d.z, sc.dir.x, sc.dir.y,
-                                                  sc.dir.z, sc.pmix, sc.pb, lum(v3(Lp.x, Lp.y, Lp.z)), lum(beta_arr),
-                                                  sc.qrr, 0.0f, GA.tag};
-                  for (int k = 0; k < kRecFloats; k++) r[k] = vals[k];
-                }
-              } else if (sc.neeKind == 1) {
-                Vec3 fdummy;
-                surface_bsdf(S.objs[sc.neeObj], hn, d, sc.dir, russian, fdummy, lastPdf);
-              } else {
-                lastPdf = sc.dirProb;
-              }
-              B.lastp[p] = f4(sc.x, 0.0f);
-            } else if (!sc.transparent) {
-              flags = 0;
-            }
-            const uint32_t nguided = ((uint32_t)s.w >> kGuidedShift) + (sc.guided ? 1u : 0u);
-            flags = (flags & ((1u << kGuidedShift) - 1u)) | (nguided << kGuidedShift);
-            B.o[p] = f4(sc.x, 0.0f);
-            B.d[p] = f4(sc.dir, 0.0f);
-            B.s[p] = make_int4(sc.inside, sc.contact, sc.contactPos, (int)flags);
-            B.beta[p] = f4(beta, lastPdf);
-            cont = true;
-          }
-        }
-      }
+      shade_path<MODE, BATCH, VOL>(S0, B, P, b, GA, BA, p, i, lds, rays, cont, nee);
     }
     staged_push(qc, cont, p);
     if (MODE >= 1) staged_push(qn, nee, p);
@@ -1284,6 +1297,82 @@ __global__ void __launch_bounds__(kRBlock, (MODE == 0 && !VOL) ? RX_SHADE_MIN_WA
     k_shade(DevScene S0, PathBufs B, PassArgs P, uint32_t b, GuideArgs GA, BatchArgs BA) {
   __shared__ ShadeLds<MODE> sh;
   shade_body<MODE, BATCH, VOL>(S0, B, P, b, GA, BA, sh);
+}
+
+// ------------------------------------------------------------------ tail
+// The last bounces of plain path tracing (RX_MODE_PT, no media, one scene) in
+// one launch.  The queue of bounce b0 thins out after Russian roulette, and a
+// bounce launch then lasts as long as its slowest ray's chain of dependent
+// node fetches whatever the ray count; launched per bounce, those slowest
+// rays add up.  Here a lane takes a path of the bounce-b0 queue and carries
+// it through its remaining bounces: trace (trip by trip, as the extend
+// kernel), then shade_path, then trace its continuation, until the path
+// ends; idle lanes take the next path.  Lanes whose trace is done wait until
+// kTailShadeMin of the wave are waiting, or none is still tracing, and shade
+// together.  Hits, random numbers (keyed by pixel, sample and bounce) and
+// the path state are the per-bounce kernels', so the film is bit-identical.
+#ifndef RX_TAIL_SHADE_MIN
+#define RX_TAIL_SHADE_MIN 32
+#endif
+constexpr int kTailShadeMin = RX_TAIL_SHADE_MIN;
+constexpr uint32_t kDefaultTailBounce = 3;  // rx_render_tuning.tail_bounce default
+
+static __global__ void __launch_bounds__(kRBlock) k_tail(DevScene S, PathBufs B, PassArgs P, uint32_t b0) {
+  __shared__ uint2 stack[kLdsDepth * kRBlock];
+  const uint32_t n = B.cnt[b0];
+  const uint32_t* q = (b0 & 1) ? B.q1 : B.q0;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint2* const lds = stack + threadIdx.x;
+  uint2* const spill = B.spill + (size_t)gtid * B.spill_depth;
+  const GuideArgs GA{};
+  const BatchArgs BA{};
+  bool ovf = false;
+  int state = 0;  // 0 idle, 1 tracing bounce b of path p, 2 its hit waits to be shaded
+  uint32_t p = 0, b = 0, traced = 0, rays = 0, segs = 0;
+  WaveWork w = wave_work(n);
+  TraceState ts;
+  TraceStats st{0u, 0u};
+  auto begin = [&]() {  // bounce b of path p, from the state the last shade wrote
+    traced++;
+    const int4 s4 = B.s[p];
+    if (trace_begin<false>(tree_root(S.tree), ts, xyz(B.o[p]), xyz(B.d[p]), s4.x, s4.y, &st)) return 1;
+    B.hit[p] = make_int2(__float_as_int(-1.0f), -1);  // misses the scene box
+    return 2;
+  };
+  for (;;) {
+    const unsigned long long idle = __ballot(state == 0);
+    if (idle && !w.drained) {
+      const uint32_t i = wave_take(w, idle, n, B.cnt + kWorkBase + b0 * kWorkWords);
+      if (i < n) {
+        p = q[i];
+        b = b0;
+        state = begin();
+      }
+    }
+    const unsigned long long tracing = __ballot(state == 1), waiting = __ballot(state == 2);
+    if (!tracing && !waiting && w.drained) break;
+    if (waiting && (!tracing || __popcll(waiting) >= kTailShadeMin)) {
+      if (state == 2) {
+        bool cont = false, nee = false;
+        segs++;
+        shade_path<0, false, false>(S, B, P, b, GA, BA, p, 0, lds, rays, cont, nee);
+        if (cont) {
+          b++;
+          state = begin();
+        } else {
+          state = 0;
+        }
+      }
+    } else if (state == 1 && trace_trip<false, true, false, kLdsDepth>(S.tree, ts, lds, spill, B.spill_depth, &st,
+                                                                       &ovf)) {
+      B.hit[p] = ts.has ? make_int2(__float_as_int(ts.best), ts.bestPos) : make_int2(__float_as_int(-1.0f), -1);
+      state = 2;
+    }
+  }
+  if (ovf) *B.overflow = 1;
+  wave_add(&B.stats[0], traced + rays);
+  wave_add(&B.stats[1], traced);
+  wave_add(&B.stats[3], segs);
 }
 
 // ------------------------------------------------------------------ shadow

@@ -338,6 +338,11 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ext, shd, sdw;
   uint64_t launches = 0, train_rows = 0;
   uint32_t pass = 0;
+  // RX_MODE_PT without media: the bounces from tail_b on run as one k_tail launch
+  // (the counting render keeps one counted extend launch per bounce)
+  const uint32_t tail_b = (mode == RX_MODE_PT && !vol && !count)
+                              ? std::max(1u, c->tuning.tail_bounce ? c->tuning.tail_bounce : kDefaultTailBounce)
+                              : ~0u;
   for (uint32_t done = 0; done < prm->spp; pass++) {
     P.sample = prm->sample_offset + done;
     // a training pass renders up to s_train samples per pixel (to the next pass boundary); otherwise up to
@@ -352,6 +357,13 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
     if (vol) hipLaunchKernelGGL((k_generate<false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, BA0);
     else hipLaunchKernelGGL((k_generate<false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, C, B, P, BA0);
     for (uint32_t b = 0; b <= prm->max_depth; b++) {
+      if (b == tail_b) {  // bounces b.. to the paths' ends in one launch, timed as an extend launch
+        hipEvent_t e0 = timing ? mark() : nullptr;
+        hipLaunchKernelGGL(k_tail, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b);
+        launches++;
+        if (timing) ext.push_back({e0, mark()});
+        break;
+      }
       hipEvent_t e0 = timing ? mark() : nullptr;
       const uint32_t sdep = c->tuning.stack_depth, sblk = c->tuning.block_size;
       if (b == 0) {
@@ -522,7 +534,8 @@ int rx_autotune_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_par
     return RX_OK;
   };
   // coordinate search: path slots x blocks per CU at the default extend
-  // shape, then the extend kernel's (stack depth, block size) at the winner
+  // shape, then the extend kernel's (stack depth, block size) at the winner,
+  // the camera-ray form and, for RX_MODE_PT, the tail launch's first bounce
   const uint32_t slot_opts[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
   const int blk_opts[] = {occ, std::max(2, (3 * occ) / 4)};
   for (uint32_t sl : slot_opts) {
@@ -546,6 +559,14 @@ int rx_autotune_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_par
     rx_render_tuning t = base;
     t.camera_rays = 1;
     if (int r = trial(t)) return r;
+  }
+  if (p.mode == RX_MODE_PT) {  // then the first bounce of the one-launch tail (64: none)
+    base = win;
+    for (uint32_t tb : {2u, 4u, 64u}) {
+      rx_render_tuning t = base;
+      t.tail_bounce = tb;
+      if (int r = trial(t)) return r;
+    }
   }
   c->tuning = win;
   if (best) *best = win;

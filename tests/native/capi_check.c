@@ -24,8 +24,9 @@ _Static_assert(sizeof(rx_camera_desc) == 48 && offsetof(rx_camera_desc, sight_di
 _Static_assert(sizeof(rx_render_params) == 32 && offsetof(rx_render_params, seed) == 16 &&
                    offsetof(rx_render_params, flags) == 28,
                "rx_render_params layout");
-_Static_assert(sizeof(rx_render_tuning) == 24 && offsetof(rx_render_tuning, stack_depth) == 12 &&
-                   offsetof(rx_render_tuning, block_size) == 16 && offsetof(rx_render_tuning, camera_rays) == 20,
+_Static_assert(sizeof(rx_render_tuning) == 28 && offsetof(rx_render_tuning, stack_depth) == 12 &&
+                   offsetof(rx_render_tuning, block_size) == 16 && offsetof(rx_render_tuning, camera_rays) == 20 &&
+                   offsetof(rx_render_tuning, tail_bounce) == 24,
                "rx_render_tuning layout");
 _Static_assert(sizeof(rx_render_stats) == 112 && offsetof(rx_render_stats, ms_total) == 56 &&
                    offsetof(rx_render_stats, extend_launches) == 88 && offsetof(rx_render_stats, guide_rows) == 96,

@@ -202,6 +202,31 @@ def test_camera_packets_full_size(rx, gpu_ctx):
     assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sumsq"], b["sumsq"])
 
 
+@pytest.mark.parametrize("name", ["icosphere", "glass"])
+def test_tail_launch_gives_identical_images(rx, gpu_ctx, name):
+    """Plain path tracing with the last bounces of every path carried by one
+    launch (k_tail, from rx_render_tuning.tail_bounce on) against one extend
+    and one shade launch per bounce: the films are bit-identical and every
+    ray is traced once, on the 1,003,526-triangle icosphere and on the glass
+    scene (paths inside the refractive sphere)."""
+    print(f"[tail] {name}", flush=True)
+    sc = scenes.icosphere_scene(256, 256, max_depth=10) if name == "icosphere" else scenes.glass_scene(64, 64)
+    pt = rx.PathTracerHIP(sc, ctx=gpu_ctx)
+    try:
+        pt.set_tuning(tail_bounce=64)  # above max_depth: no tail launch
+        a = pt.renderPixels(spp=8, mode=MODE_PT, seed=5)
+        for tb in [0, 1, 2, 5]:
+            pt.set_tuning(tail_bounce=tb)
+            b = pt.renderPixels(spp=8, mode=MODE_PT, seed=5)
+            print(f"  tail from bounce {tb or 'default'}: {b['stats']['ms_total']:.2f} ms "
+                  f"(per-bounce launches {a['stats']['ms_total']:.2f} ms)")
+            assert b["stats"]["rays"] == a["stats"]["rays"], tb
+            assert b["stats"]["segments"] == a["stats"]["segments"], tb
+            assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sumsq"], b["sumsq"]), tb
+    finally:
+        pt.set_tuning()
+
+
 def test_launch_tuning_gives_identical_images(rx, gpu_ctx):
     """rx_render_tuning (samples per pass, resident blocks, the extend
     kernel's stack depth and block size) and the autotuner change only the
