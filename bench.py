@@ -468,9 +468,13 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / args.steps
     value = rays / wall
-    # roofline of the dominant kernel (extend): algorithmic bytes per launch / avg duration
+    # roofline of the dominant kernel (extend): algorithmic bytes of a step's traversal (the counting
+    # render's canonical box and triangle tests) over the step's extend time.  The timed render runs
+    # fewer, larger extend launches than the counting one (bounces from the tail bounce on are one
+    # k_tail launch), so the bytes per launch are the step's over the timed launches.
     bytes_step = B_RAY * cs["extend_rays"] + B_BOX * cs["box_tests"] + B_TRI * cs["tri_tests"]
-    per_launch_bytes = bytes_step / max(1, cnt["stats"]["extend_launches"])
+    launches_step = max(1, ext_launches) / max(1, args.steps)
+    per_launch_bytes = bytes_step / launches_step
     avg_launch_s = ext_ms / 1000.0 / max(1, ext_launches)
     achieved = per_launch_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     out = {
@@ -492,7 +496,8 @@ def main():
         "samples_per_s": args.res * args.res * args.spp * args.steps / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": EXTEND_TRAFFIC_BYTES,
-                     "kernel": "extend (nearest-hit traversal: k_extend_packet for camera rays, k_extend after)",
+                     "kernel": "extend (nearest-hit traversal: k_extend_packet for camera rays, k_extend for "
+                               "bounces 1-2, k_tail for bounces 3-10 with their shading)",
                      "note": "achieved counts the canonical traversal's algorithmic bytes; the caches serve about 90 % "
                              "of them (traffic is the measured HBM bytes per launch), so frac can exceed 1; the "
                              "kernels are bound by VALU issue",

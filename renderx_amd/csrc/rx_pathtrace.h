@@ -1218,23 +1218,27 @@ __device__ __forceinline__ void shade_path(const DevScene& S0, const PathBufs& B
         float lastPdf = be.w;
         if (sc.random) {
           flags = FLAG_MIS;
-          if (sc.guided) {
-            lastPdf = sc.pdfB;
-            if (GA.tag > 0.0f) {  // training record of this vertex
-              float* r = GA.rec + ((size_t)p * GA.R + b) * kRecFloats;
-              const float4 Lp = B.L[p];
-              const float vals[kRecFloats] = {sc.x.x, sc.x.y, sc.x.z, -d.x, -d.y, -d.z, sc.dir.x, sc.dir.y,
-                                              sc.dir.z, sc.pmix, sc.pb, lum(v3(Lp.x, Lp.y, Lp.z)), lum(beta_arr),
-                                              sc.qrr, 0.0f, GA.tag};
-              for (int k = 0; k < kRecFloats; k++) r[k] = vals[k];
+          // the density the direction was drawn with and the vertex, for the MIS
+          // weight of an emitter the continuation hits: next-event estimation only
+          if (MODE >= 1) {
+            if (sc.guided) {
+              lastPdf = sc.pdfB;
+              if (GA.tag > 0.0f) {  // training record of this vertex
+                float* r = GA.rec + ((size_t)p * GA.R + b) * kRecFloats;
+                const float4 Lp = B.L[p];
+                const float vals[kRecFloats] = {sc.x.x, sc.x.y, sc.x.z, -d.x, -d.y, -d.z, sc.dir.x, sc.dir.y,
+                                                sc.dir.z, sc.pmix, sc.pb, lum(v3(Lp.x, Lp.y, Lp.z)), lum(beta_arr),
+                                                sc.qrr, 0.0f, GA.tag};
+                for (int k = 0; k < kRecFloats; k++) r[k] = vals[k];
+              }
+            } else if (sc.neeKind == 1) {
+              Vec3 fdummy;
+              surface_bsdf(S.objs[sc.neeObj], hn, d, sc.dir, russian, fdummy, lastPdf);
+            } else {
+              lastPdf = sc.dirProb;
             }
-          } else if (sc.neeKind == 1) {
-            Vec3 fdummy;
-            surface_bsdf(S.objs[sc.neeObj], hn, d, sc.dir, russian, fdummy, lastPdf);
-          } else {
-            lastPdf = sc.dirProb;
+            B.lastp[p] = f4(sc.x, 0.0f);
           }
-          B.lastp[p] = f4(sc.x, 0.0f);
         } else if (!sc.transparent) {
           flags = 0;
         }
