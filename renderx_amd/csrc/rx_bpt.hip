@@ -536,56 +536,64 @@ __device__ __forceinline__ BVert link_to(BVert src, const BVert& dst, int ci, in
   src.inside = dst.inside;
   return src;
 }
-// connectWeight (:197-263) with expTerm = 1, in double as the reference; the
-// backward densities first, then the forward ones accumulated on the fly.
-// Each loop slides a window of three neighbouring vertices along the path,
-// so every vertex is read once per loop; the forward loop also finds whether
-// any two neighbours are both RANDOM (mustUsePT otherwise).
+// connectWeight (:197-263) with expTerm = 1, in double as the reference.  The
+// reference fills the backward densities pb[i] (the path's density with the
+// light subpath starting at vertex i) and then sums pf[i] pb[i + 1] over the
+// neighbours i, i + 1 that are both RANDOM, pf[i] the forward densities
+// (mustUsePT: pb[0] when there is no such pair).  With pb[i - 1] = pb[i] G_i
+// and pf[i + 1] = pf[i] F_i the sum is pf[0] T_0 for the recurrence
+//   T_j = [pair at j] pb[j + 1] + F_j T_{j + 1},   T_{W-1} = 0,
+// and the strategy's own term pf[ci] pb[ci + 1] = pf[0] U_0 with U_j =
+// [j == ci] pb[j + 1] + F_j U_{j + 1}; one backward pass carries pb, T and U
+// in registers (products and sums only, as the reference), sliding a window
+// of four vertices j - 1 .. j + 2, so every vertex is read once.
 __device__ float connect_weight(const DevScene& S, const BView& v, int ci) {
   const int W = v.W;
-  double pb[2 * kBptMaxVerts];
-  BVert a{}, b = v.get(W - 1), c;  // vertices i + 1, i, i - 1
-  pb[W - 1] = b.originProb;
-  for (int i = W - 1; i > 0; i--) {
-    c = v.get(i - 1);
+  // G_i (vertices i + 1, i, i - 1 as a, b, c) applied to x
+  auto back = [&](double x, const BVert& a, const BVert& b, const BVert& c, int i) {
     const float dl = fmaxf(length(b.o - c.o), 1e-5f);
-    pb[i - 1] = pb[i] * (double)bpt_cos(S, link_to(c, b, ci, i - 1, i));
-    pb[i - 1] /= (double)dl * (double)dl;
+    double r = x * (double)bpt_cos(S, link_to(c, b, ci, i - 1, i));
+    r /= (double)dl * (double)dl;
     if (b.random) {
       const BVert bc = link_to(b, c, ci, i, i - 1);
-      if (i < W - 1) pb[i - 1] *= (double)bpt_pdf(S, link_to(a, b, ci, i + 1, i).d, bc);
-      else pb[i - 1] *= (double)self_pdf(bc);
-      pb[i - 1] *= (double)origin_pdf(S, bc, c, dl);
+      if (i < W - 1) r *= (double)bpt_pdf(S, link_to(a, b, ci, i + 1, i).d, bc);
+      else r *= (double)self_pdf(bc);
+      r *= (double)origin_pdf(S, bc, c, dl);
     }
-    a = b;
-    b = c;
-  }
-  a = BVert{};
-  b = v.get(0);  // vertices i - 1, i, i + 1
-  double pf = b.originProb, sum = 0.0, selfF = 0.0;
-  bool any_pair = false;
-  for (int i = 0; i < W - 1; i++) {
-    if (i == ci) selfF = pf;
-    c = v.get(i + 1);
-    if (b.random && c.random) {
-      sum += pf * pb[i + 1];
-      any_pair = true;
-    }
+    return r;
+  };
+  // F_j (vertices j - 1, j, j + 1 as a, b, c)
+  auto fwd = [&](const BVert& a, const BVert& b, const BVert& c, int j) {
     const float dl = fmaxf(length(c.o - b.o), 1e-5f);
-    double nf = pf * (double)bpt_cos(S, link_to(c, b, ci, i + 1, i));
-    nf /= (double)dl * (double)dl;
+    double r = (double)bpt_cos(S, link_to(c, b, ci, j + 1, j));
+    r /= (double)dl * (double)dl;
     if (b.random) {
-      const BVert bc = link_to(b, c, ci, i, i + 1);
-      if (i > 0) nf *= (double)bpt_pdf(S, link_to(a, b, ci, i - 1, i).d, bc);
-      else nf *= (double)self_pdf(bc);
-      nf *= (double)origin_pdf(S, bc, c, dl);
+      const BVert bc = link_to(b, c, ci, j, j + 1);
+      if (j > 0) r *= (double)bpt_pdf(S, link_to(a, b, ci, j - 1, j).d, bc);
+      else r *= (double)self_pdf(bc);
+      r *= (double)origin_pdf(S, bc, c, dl);
     }
-    pf = nf;
-    a = b;
-    b = c;
+    return r;
+  };
+  BVert d{}, c = v.get(W - 1), b = W > 1 ? v.get(W - 2) : c;  // vertices j + 2, j + 1, j
+  double pb = c.originProb;                                    // pb[j + 1]
+  double T = 0.0, U = 0.0;
+  bool any_pair = false;
+  for (int j = W - 2; j >= 0; j--) {
+    const BVert a = j > 0 ? v.get(j - 1) : BVert{};
+    const bool pair = b.random && c.random;
+    any_pair |= pair;
+    const double F = fwd(a, b, c, j);
+    T = (pair ? pb : 0.0) + F * T;
+    U = (j == ci ? pb : 0.0) + F * U;
+    pb = back(pb, d, c, b, j + 1);  // pb[j]
+    d = c;
+    c = b;
+    b = a;
   }
-  if (!any_pair) sum += pb[0];  // mustUsePT
-  const double self = ci == -1 ? pb[0] : selfF * pb[ci + 1];
+  const double pf0 = c.originProb;  // vertex 0
+  const double sum = pf0 * T + (any_pair ? 0.0 : pb);
+  const double self = ci == -1 ? pb : pf0 * U;
   const double res = self / sum;
   if (!(res >= 0)) return 0.0f;
   return (float)res;
