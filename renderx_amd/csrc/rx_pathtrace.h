@@ -908,12 +908,13 @@ __device__ void surface_scatter(const DevScene& S, int obj, Vec3 x, Vec3 d, Vec3
   }
 }
 
-// minimum resident waves per SIMD for the plain medium-free shade kernel
-// (97 VGPRs unconstrained; 6 waves spill 15 and beat 4 and 5 by about 0.5 %
-// per bench step).  The other variants need far more registers and keep
-// their own allocation.
+// minimum resident waves per SIMD for the plain medium-free shade kernel:
+// 5 (96 VGPRs, a 12-byte spill) shades a 64-spp 1024^2 bench render in
+// 4.68 ms against 4.79 at 4, 4.82 at 8 and 5.30 at 6 (80 VGPRs, a 68-byte
+// spill).  The other variants need far more registers and keep their own
+// allocation.
 #ifndef RX_SHADE_MIN_WAVES
-#define RX_SHADE_MIN_WAVES 6
+#define RX_SHADE_MIN_WAVES 5
 #endif
 // One scatter of samplePathIter (src/MCRenderer.cpp:151-267) after the hit
 // of the ray (o, d) is known (t, pos, hitObj, normal hn; inside: the object
