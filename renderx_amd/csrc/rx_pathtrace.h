@@ -23,9 +23,12 @@ constexpr int NONE = -1, CAMERA = -2;
 static_assert(kNoCondition != NONE && kNoCondition != CAMERA, "distinct contact sentinels");
 constexpr int kRBlock = kTraceBlock;  // render kernels share the traversal block shape
 #ifndef RX_EXTEND_MIN_WAVES
-#define RX_EXTEND_MIN_WAVES 6
+#define RX_EXTEND_MIN_WAVES 7
 #endif
-// minimum resident waves per SIMD the extend kernel's registers must allow
+// minimum resident waves per SIMD the extend kernel's registers must allow:
+// 7 (72 VGPRs, a 44-byte spill at the ray set-up; 14 blocks of 10 KB of LDS
+// stack per CU) traces the bench render's bounces in 27.2 ms against 27.9 at
+// 6 (80 VGPRs), 27.9 at 8 and 29.3 at 5
 constexpr int kExtendMinWaves = RX_EXTEND_MIN_WAVES;
 #ifndef RX_SHADOW_MIN_WAVES
 #define RX_SHADOW_MIN_WAVES 4

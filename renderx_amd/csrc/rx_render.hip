@@ -218,6 +218,14 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   const int per_cu = c->tuning.blocks_per_cu ? std::min<int>((int)c->tuning.blocks_per_cu, c->render_blocks_per_cu)
                                              : c->render_blocks_per_cu;
   const int blocks = (int)std::min<uint64_t>((N + kRBlock - 1) / kRBlock, (uint64_t)c->n_cu * per_cu);
+  // the shade and shadow kernels take any grid (chunks of their queue), so
+  // theirs is what they hold resident (they need more registers than the
+  // extend kernel): a larger grid would run a second, partial round of blocks
+  auto resident = [&](const void* k) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kRBlock, 0) != hipSuccess || nb < 1) nb = per_cu;
+    return (int)std::min<uint64_t>((N + kRBlock - 1) / kRBlock, (uint64_t)c->n_cu * std::min(nb, per_cu));
+  };
   int sd = 0;
   if (int r = ensure_spill(c, blocks, kRBlock, (int)R.scene.max_depth, &sd)) return r;
   PathBufs B;
@@ -340,6 +348,13 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   uint32_t pass = 0;
   // RX_MODE_PT without media: the bounces from tail_b on run as one k_tail launch
   // (the counting render keeps one counted extend launch per bounce)
+  const void* shade_fn = mode == RX_MODE_GUIDED ? (vol ? (const void*)k_shade<2, false, true> : (const void*)k_shade<2, false, false>)
+                         : mode == RX_MODE_NEE   ? (vol ? (const void*)k_shade<1, false, true> : (const void*)k_shade<1, false, false>)
+                                                 : (vol ? (const void*)k_shade<0, false, true> : (const void*)k_shade<0, false, false>);
+  const int sblocks = resident(shade_fn);
+  // the shadow kernel's waves take their first chunk by wave index: a wave
+  // that is not resident yet would hold its chunk back until a second round
+  const int shblocks = resident(vol ? (const void*)k_shadow<false, true> : (const void*)k_shadow<false, false>);
   const uint32_t tail_b = (mode == RX_MODE_PT && !vol && !count)
                               ? std::max(1u, c->tuning.tail_bounce ? c->tuning.tail_bounce : kDefaultTailBounce)
                               : ~0u;
@@ -382,20 +397,20 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
         RXCK(guide_infer(GA.net, GA.query, GA.query + 1, 2, B.cnt + kGuideRowBase + b, P.n_paths, nullptr, GA.head,
                          c->n_cu, s),
              "guide inference");
-        if (vol) hipLaunchKernelGGL((k_shade<2, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
-        else hipLaunchKernelGGL((k_shade<2, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        if (vol) hipLaunchKernelGGL((k_shade<2, false, true>), dim3(sblocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        else hipLaunchKernelGGL((k_shade<2, false, false>), dim3(sblocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
       } else if (mode == RX_MODE_NEE) {
-        if (vol) hipLaunchKernelGGL((k_shade<1, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
-        else hipLaunchKernelGGL((k_shade<1, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        if (vol) hipLaunchKernelGGL((k_shade<1, false, true>), dim3(sblocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        else hipLaunchKernelGGL((k_shade<1, false, false>), dim3(sblocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
       } else {
-        if (vol) hipLaunchKernelGGL((k_shade<0, false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
-        else hipLaunchKernelGGL((k_shade<0, false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        if (vol) hipLaunchKernelGGL((k_shade<0, false, true>), dim3(sblocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        else hipLaunchKernelGGL((k_shade<0, false, false>), dim3(sblocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
       }
       hipEvent_t e2 = timing ? mark() : nullptr;
       hipEvent_t e3 = nullptr;
       if (mode) {
-        if (vol) hipLaunchKernelGGL((k_shadow<false, true>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
-        else hipLaunchKernelGGL((k_shadow<false, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        if (vol) hipLaunchKernelGGL((k_shadow<false, true>), dim3(shblocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
+        else hipLaunchKernelGGL((k_shadow<false, false>), dim3(shblocks), dim3(kRBlock), 0, s, S, B, P, b, GA, BA0);
         e3 = timing ? mark() : nullptr;
       }
       if (timing) {
