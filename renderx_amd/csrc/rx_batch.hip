@@ -311,6 +311,15 @@ int batch_render(rx_batch* b, rx_render_stats* stats, uint32_t max_passes) {
   const int per_cu = c->tuning.blocks_per_cu ? std::min<int>((int)c->tuning.blocks_per_cu, c->render_blocks_per_cu)
                                              : c->render_blocks_per_cu;
   const int blocks = (int)std::min<uint64_t>((N + kRBlock - 1) / kRBlock, (uint64_t)c->n_cu * per_cu);
+  // shade and shadow take any grid; theirs is what they hold resident (see rx_render)
+  auto resident = [&](const void* k) {
+    int nr = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nr, k, kRBlock, 0) != hipSuccess || nr < 1) nr = per_cu;
+    return (int)std::min<uint64_t>((N + kRBlock - 1) / kRBlock, (uint64_t)c->n_cu * std::min(nr, per_cu));
+  };
+  const int sblocks = resident(b->mode == RX_MODE_NEE ? (const void*)k_shade<1, true, false>
+                                                     : (const void*)k_shade<0, true, false>);
+  const int shblocks = resident((const void*)k_shadow<true, false>);
   int sd = 0;
   if (int r = ensure_spill(c, blocks, kRBlock, (int)b->max_tree_depth, &sd)) return r;
   PathBufs B;
@@ -367,10 +376,10 @@ int batch_render(rx_batch* b, rx_render_stats* stats, uint32_t max_passes) {
       else if (bb == 0) launch_extend<false, false, true>(sdep, sblk, blocks, s, S.tree, B, bb, BA);
       else launch_extend<true, false, true>(sdep, sblk, blocks, s, S.tree, B, bb, BA);
       if (b->mode == RX_MODE_NEE) {
-        hipLaunchKernelGGL((k_shade<1, true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
-        hipLaunchKernelGGL((k_shadow<true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
+        hipLaunchKernelGGL((k_shade<1, true, false>), dim3(sblocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
+        hipLaunchKernelGGL((k_shadow<true, false>), dim3(shblocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
       } else {
-        hipLaunchKernelGGL((k_shade<0, true, false>), dim3(blocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
+        hipLaunchKernelGGL((k_shade<0, true, false>), dim3(sblocks), dim3(kRBlock), 0, s, S, B, P, bb, GA, BA);
       }
     }
     hipLaunchKernelGGL(k_accumulate_batch, dim3(blocks), dim3(kRBlock), 0, s, B, BA);
