@@ -140,7 +140,16 @@ typedef struct rx_render_stats {
   uint64_t extend_launches;
   uint64_t guide_rows;       /* guided mode: network queries evaluated (one per guided vertex) */
   uint64_t guide_train_rows; /* guided mode: training rows over all Adam steps (batch_size per step) */
+  /* per-bounce breakdown, summed over the render's passes; bounce b >= RX_STATS_BOUNCES-1 adds to the
+     last entry.  RX_RENDER_TIMING: time of the extend launch that starts at bounce b (k_tail's whole
+     launch, traversal and shading of every later bounce, at its first bounce).  RX_RENDER_COUNT_WORK:
+     the counted extend launch's nearest-hit queries, box tests and triangle tests at bounce b. */
+  double ms_extend_bounce[16];
+  uint64_t extend_rays_bounce[16];
+  uint64_t box_tests_bounce[16];
+  uint64_t tri_tests_bounce[16];
 } rx_render_stats;
+#define RX_STATS_BOUNCES 16
 
 typedef struct rx_ctx rx_ctx;
 typedef struct rx_tree rx_tree;

@@ -24,7 +24,9 @@ RENDER_PARAMS = np.dtype([("spp", "<u4"), ("max_depth", "<u4"), ("rr_start_depth
 RENDER_STATS = np.dtype([("rays", "<u8"), ("extend_rays", "<u8"), ("shadow_rays", "<u8"), ("segments", "<u8"),
                          ("samples", "<u8"), ("box_tests", "<u8"), ("tri_tests", "<u8"), ("ms_total", "<f8"),
                          ("ms_extend", "<f8"), ("ms_shade", "<f8"), ("ms_shadow", "<f8"),
-                         ("extend_launches", "<u8"), ("guide_rows", "<u8"), ("guide_train_rows", "<u8")])
+                         ("extend_launches", "<u8"), ("guide_rows", "<u8"), ("guide_train_rows", "<u8"),
+                         ("ms_extend_bounce", "<f8", (16,)), ("extend_rays_bounce", "<u8", (16,)),
+                         ("box_tests_bounce", "<u8", (16,)), ("tri_tests_bounce", "<u8", (16,))])
 RENDER_TIMING, RENDER_COUNT_WORK = 1, 2
 RENDER_TUNING = np.dtype([("path_slots", "<u4"), ("blocks_per_cu", "<u4"), ("use_graph", "<u4"), ("stack_depth", "<u4"),
                           ("block_size", "<u4"), ("camera_rays", "<u4"), ("tail_bounce", "<u4")])
@@ -202,8 +204,9 @@ class PathTracerHIP:
         return {k: int(t[k][0]) for k in RENDER_TUNING.names}, float(ms[0])
 
     def renderPixels(self, camera=None, spp=None, max_depth=None, rr_start_depth=None, mode=MODE_PT, seed=1,
-                     sample_offset=0, want_sums=True, flags=0):
-        """Returns dict(mean=(H,W,3) f32, sum, sumsq=(H,W,3) f64, stats)."""
+                     sample_offset=0, want_sums=True, flags=0, want_mean=True):
+        """Returns dict(mean=(H,W,3) f32, sum, sumsq=(H,W,3) f64, stats).  With want_sums and want_mean
+        false the film stays on the device (no host transfer; mean, sum and sumsq are None)."""
         cam = np.ascontiguousarray((self.scene.camera if camera is None else camera).reshape(1))
         H, W = int(cam["height"][0]), int(cam["width"][0])
         p = np.zeros(1, RENDER_PARAMS)
@@ -214,13 +217,13 @@ class PathTracerHIP:
         p["seed"] = seed
         p["sample_offset"] = sample_offset
         p["flags"] = flags
-        mean = np.empty((H, W, 3), np.float32)
+        mean = np.empty((H, W, 3), np.float32) if want_mean else None
         s = np.empty((H, W, 3), np.float64) if want_sums else None
         q = np.empty((H, W, 3), np.float64) if want_sums else None
         st = np.zeros(1, RENDER_STATS)
         self.ctx.check(lib().rx_render(self.ctx.h, ptr(cam), ptr(p), ptr(mean), ptr(s), ptr(q), ptr(st)),
                        "rx_render")
-        return {"mean": mean, "sum": s, "sumsq": q, "stats": {k: st[k][0].item() for k in RENDER_STATS.names}}
+        return {"mean": mean, "sum": s, "sumsq": q, "stats": {k: st[k][0].tolist() if st[k].ndim > 1 else st[k][0].item() for k in RENDER_STATS.names}}
 
 
 class BidirectionalPathTracerHIP(PathTracerHIP):
@@ -289,7 +292,7 @@ class BatchRenderer:
     def render(self):
         st = np.zeros(1, RENDER_STATS)
         self.ctx.check(lib().rx_batch_render(self.h, ptr(st)), "rx_batch_render")
-        return {k: st[k][0].item() for k in RENDER_STATS.names}
+        return {k: st[k][0].tolist() if st[k].ndim > 1 else st[k][0].item() for k in RENDER_STATS.names}
 
     def image(self, k):
         H, W = self.shapes[k]

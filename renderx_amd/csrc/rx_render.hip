@@ -17,8 +17,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <new>
 #include <vector>
 
@@ -344,6 +346,11 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
   RXCK(hipGetLastError(), "launch fill");
   RXCK(hipEventRecord(c->ev0, s), "event");
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ext, shd, sdw;
+  std::vector<uint32_t> ext_b;  // bounce of each timed extend launch
+  // counting render: the work counters after each counted extend launch (differences give the bounce's work)
+  // (a deque: an element keeps its address while its copy is in flight)
+  std::deque<std::array<unsigned long long, 8>> snaps;
+  std::vector<uint32_t> snap_b;
   uint64_t launches = 0, train_rows = 0;
   uint32_t pass = 0;
   // RX_MODE_PT without media: the bounces from tail_b on run as one k_tail launch
@@ -376,7 +383,10 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
         hipEvent_t e0 = timing ? mark() : nullptr;
         hipLaunchKernelGGL(k_tail, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, b);
         launches++;
-        if (timing) ext.push_back({e0, mark()});
+        if (timing) {
+          ext.push_back({e0, mark()});
+          ext_b.push_back(b);
+        }
         break;
       }
       hipEvent_t e0 = timing ? mark() : nullptr;
@@ -390,6 +400,11 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
         else launch_extend<true, false, false>(sdep, sblk, blocks, s, S.tree, B, b, BA0);
       }
       launches++;
+      if (count) {  // pageable destination: the copy is ordered on the stream and read after the final sync
+        snaps.emplace_back();
+        snap_b.push_back(b);
+        RXCK(hipMemcpyAsync(snaps.back().data(), B.stats, 64, hipMemcpyDeviceToHost, s), "download counters");
+      }
       hipEvent_t e1 = timing ? mark() : nullptr;
       if (mode == RX_MODE_GUIDED) {  // network inference for this bounce's vertices
         if (vol) hipLaunchKernelGGL(k_guide_query<true>, dim3(blocks), dim3(kRBlock), 0, s, S, B, P, GA, b);
@@ -415,6 +430,7 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       }
       if (timing) {
         ext.push_back({e0, e1});
+        ext_b.push_back(b);
         shd.push_back({e1, e2});
         if (mode) sdw.push_back({e2, e3});
       }
@@ -487,6 +503,20 @@ int rx_render(rx_ctx* c, const rx_camera_desc* cam, const rx_render_params* prm,
       return t;
     };
     stats->ms_extend = tsum(ext);
+    const uint32_t lastb = RX_STATS_BOUNCES - 1;
+    for (size_t i = 0; i < ext.size(); i++) {
+      float m = 0;
+      if (ext[i].first && ext[i].second && hipEventElapsedTime(&m, ext[i].first, ext[i].second) == hipSuccess)
+        stats->ms_extend_bounce[std::min(ext_b[i], lastb)] += m;
+    }
+    std::array<unsigned long long, 8> prev{};
+    for (size_t i = 0; i < snaps.size(); i++) {
+      const uint32_t b = std::min(snap_b[i], lastb);
+      stats->extend_rays_bounce[b] += snaps[i][1] - prev[1];
+      stats->box_tests_bounce[b] += snaps[i][4] - prev[4];
+      stats->tri_tests_bounce[b] += snaps[i][5] - prev[5];
+      prev = snaps[i];
+    }
     stats->ms_shade = tsum(shd);
     stats->ms_shadow = tsum(sdw);
   }

@@ -28,8 +28,12 @@ _Static_assert(sizeof(rx_render_tuning) == 28 && offsetof(rx_render_tuning, stac
                    offsetof(rx_render_tuning, block_size) == 16 && offsetof(rx_render_tuning, camera_rays) == 20 &&
                    offsetof(rx_render_tuning, tail_bounce) == 24,
                "rx_render_tuning layout");
-_Static_assert(sizeof(rx_render_stats) == 112 && offsetof(rx_render_stats, ms_total) == 56 &&
-                   offsetof(rx_render_stats, extend_launches) == 88 && offsetof(rx_render_stats, guide_rows) == 96,
+_Static_assert(sizeof(rx_render_stats) == 624 && offsetof(rx_render_stats, ms_total) == 56 &&
+                   offsetof(rx_render_stats, extend_launches) == 88 && offsetof(rx_render_stats, guide_rows) == 96 &&
+                   offsetof(rx_render_stats, ms_extend_bounce) == 112 &&
+                   offsetof(rx_render_stats, extend_rays_bounce) == 240 &&
+                   offsetof(rx_render_stats, box_tests_bounce) == 368 &&
+                   offsetof(rx_render_stats, tri_tests_bounce) == 496 && RX_STATS_BOUNCES == 16,
                "rx_render_stats layout");
 _Static_assert(sizeof(rx_guide_params) == 40, "rx_guide_params layout");
 

@@ -227,6 +227,41 @@ def test_tail_launch_gives_identical_images(rx, gpu_ctx, name):
         pt.set_tuning()
 
 
+def test_per_bounce_stats_add_up(rx, gpu_ctx):
+    """rx_render_stats' per-bounce breakdown: the counted work of each bounce
+    sums to the render's totals, bounce 0 traces one camera ray per sample,
+    and the breakdown does not depend on how the samples are split into
+    passes (one pass against four); the timed breakdown sums to ms_extend and
+    puts the tail launch at its first bounce."""
+    print("[per-bounce stats]", flush=True)
+    sc = scenes.icosphere_scene(256, 256, max_depth=10)
+    pt = rx.PathTracerHIP(sc, ctx=gpu_ctx)
+    try:
+        npaths = 256 * 256 * 8
+        counted = []
+        for slots in (1 << 22, 1 << 17):
+            pt.set_tuning(path_slots=slots)
+            st = pt.renderPixels(spp=8, mode=MODE_PT, seed=9, want_sums=False, flags=rx.RENDER_COUNT_WORK)["stats"]
+            print(f"  {slots} slots: {st['extend_launches']} launches, bounce 0 box tests {st['box_tests_bounce'][0]}")
+            assert st["extend_launches"] == 11 * ((npaths + slots - 1) // slots)
+            assert sum(st["extend_rays_bounce"]) == st["extend_rays"]
+            assert sum(st["box_tests_bounce"]) == st["box_tests"]
+            assert sum(st["tri_tests_bounce"]) == st["tri_tests"]
+            assert st["extend_rays_bounce"][0] == npaths
+            assert all(v == 0 for v in st["extend_rays_bounce"][11:])
+            counted.append(st)
+        for k in ("extend_rays_bounce", "box_tests_bounce", "tri_tests_bounce"):
+            assert counted[0][k] == counted[1][k], k
+        pt.set_tuning(tail_bounce=3)
+        st = pt.renderPixels(spp=8, mode=MODE_PT, seed=9, want_sums=False, flags=rx.RENDER_TIMING)["stats"]
+        msb = st["ms_extend_bounce"]
+        print("  ms per bounce " + " ".join(f"{m:.3f}" for m in msb[:5]))
+        assert all(m > 0 for m in msb[:4]) and all(m == 0 for m in msb[4:])
+        assert abs(sum(msb) - st["ms_extend"]) <= 1e-6 * max(1.0, st["ms_extend"])
+    finally:
+        pt.set_tuning()
+
+
 def test_launch_tuning_gives_identical_images(rx, gpu_ctx):
     """rx_render_tuning (samples per pass, resident blocks, the extend
     kernel's stack depth and block size) and the autotuner change only the
