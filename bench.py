@@ -425,7 +425,7 @@ def main():
                     help="only one graph-replayed render of the 64-scene batch (for a kernel trace); prints its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only"}))
+        print(json.dumps({"error": "renderx_amd renders on one GPU; sharding samples over several GPUs is not implemented"}))
         sys.exit(2)
     if args.guided_profile:
         print(json.dumps(guided_profile(args.guided_profile)), flush=True)

@@ -1,5 +1,5 @@
 /* A compiled C caller of librxhip.so through include/rxhip.h only, as a
- * RenderX maintainer's adapter would be: the struct layouts
+ * RenderX maintainer's adapter of IntersectionGPU would be: the struct layouts
  * the Python bindings and the adapters assume are checked at compile time,
  * then the entry points are called.
  *   capi_check cpu : host-only entry points; rx_create must report
